@@ -1,0 +1,602 @@
+/*
+ * Batched jobs and the serving loop.
+ *
+ * The reference runs one job per process and spreads its permutations over
+ * std::thread chunks (src/run/single_node.rs:83-137) or MPI ranks
+ * (src/run/multi_node.rs:47-162).  Here many jobs of uneven size share one
+ * persistent kernel (k_batch): every workgroup pulls (job, task range) items
+ * from a device queue until it is empty.  A batch's launch sequence (queue
+ * reset + persistent kernel) is captured once in a hipGraph and replayed.
+ *
+ * The server does continuous batching: requests are admitted while their HBM
+ * footprint fits the budget, every scheduling slice takes up to
+ * perms_per_slice tasks from every active request, and two slices are kept in
+ * flight on two streams so requests that arrive while others run join the
+ * next slice and overlap with them on the device.
+ */
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/dto.h"
+#include "dto_internal.h"
+#include "dto_kernels.h"
+
+using namespace dto;
+
+namespace {
+
+/* Launch shapes the batch kernels are built for. */
+template <int NT, int R, bool HL>
+hipError_t launch_batch_t(uint32_t grid, uint32_t lds, hipStream_t st, const BatchJob *jobs, const WorkItem *items,
+                          uint32_t n_items, uint64_t seed, uint32_t *counter, int *err) {
+  auto kern = k_batch<NT, R, HL>;
+  (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, jobs, items, n_items, seed, counter, err);
+  return hipGetLastError();
+}
+
+template <int NT, bool HL>
+hipError_t launch_batch_r(int rmax, uint32_t grid, uint32_t lds, hipStream_t st, const BatchJob *jobs,
+                          const WorkItem *items, uint32_t n_items, uint64_t seed, uint32_t *counter, int *err) {
+  switch (rmax) {
+    case 8: return launch_batch_t<NT, 8, HL>(grid, lds, st, jobs, items, n_items, seed, counter, err);
+    case 16: return launch_batch_t<NT, 16, HL>(grid, lds, st, jobs, items, n_items, seed, counter, err);
+    case 32: return launch_batch_t<NT, 32, HL>(grid, lds, st, jobs, items, n_items, seed, counter, err);
+    default: return launch_batch_t<NT, 64, HL>(grid, lds, st, jobs, items, n_items, seed, counter, err);
+  }
+}
+
+hipError_t launch_batch(int nt, int rmax, bool hl, uint32_t grid, uint32_t lds, hipStream_t st, const BatchJob *jobs,
+                        const WorkItem *items, uint32_t n_items, uint64_t seed, uint32_t *counter, int *err) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, counter, err);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (nt == 256)
+    return hl ? launch_batch_r<256, true>(rmax, grid, lds, st, jobs, items, n_items, seed, counter, err)
+              : launch_batch_r<256, false>(rmax, grid, lds, st, jobs, items, n_items, seed, counter, err);
+  return hl ? launch_batch_r<512, true>(rmax, grid, lds, st, jobs, items, n_items, seed, counter, err)
+            : launch_batch_r<512, false>(rmax, grid, lds, st, jobs, items, n_items, seed, counter, err);
+}
+
+/* Pair-register count for a list2 length at nt threads. */
+int rmax_for(uint32_t n2, int nt) {
+  uint32_t per = (n2 + nt - 1) / nt;
+  return per <= 8 ? 8 : per <= 16 ? 16 : per <= 32 ? 32 : per <= 64 ? 64 : -1;
+}
+
+/* Batch-wide launch parameters for a set of jobs at nt threads. */
+struct Shape {
+  int nt = 512;
+  int rmax = 8;
+  bool hl = true;
+  uint32_t lds = 0;
+  std::vector<LdsLayout> lo;
+};
+
+int plan_shape(const std::vector<dto_job *> &jobs, int nt, size_t lds_limit, Shape &s) {
+  s.nt = nt;
+  s.rmax = 8;
+  bool fit_h = true;
+  for (dto_job *j : jobs) {
+    int r = rmax_for(j->dev.N2, nt);
+    if (r < 0) return fail(DTO_ERR_TOO_LARGE, "list2 too long for this launch shape");
+    s.rmax = std::max(s.rmax, r);
+    if (make_layout(j->h, nt / 64, true).total > lds_limit) fit_h = false;
+  }
+  s.hl = fit_h;
+  s.lo.clear();
+  s.lds = 0;
+  for (dto_job *j : jobs) {
+    LdsLayout L = make_layout(j->h, nt / 64, fit_h);
+    if (L.total > lds_limit) return fail(DTO_ERR_TOO_LARGE, "job does not fit in LDS");
+    s.lo.push_back(L);
+    s.lds = std::max(s.lds, L.total);
+  }
+  return DTO_OK;
+}
+
+uint32_t grid_for(int cus, uint32_t lds, int nt, uint32_t n_items) {
+  uint32_t by_lds = std::max<uint32_t>(1, (160u * 1024u) / std::max<uint32_t>(lds, 1));
+  uint32_t by_threads = std::max<uint32_t>(1, 2048u / (uint32_t)nt);
+  uint32_t per_cu = std::min(by_lds, by_threads);
+  return std::max<uint32_t>(1, std::min<uint32_t>(n_items, (uint32_t)cus * per_cu));
+}
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------ */
+/* batch                                                                 */
+/* ------------------------------------------------------------------ */
+struct dto_batch {
+  std::vector<dto_job *> jobs;
+  std::vector<uint64_t> n_perm;
+  uint64_t seed = 0;
+  int device = -1;
+  int cus = 0;
+  size_t lds_limit = 0;
+  Shape shape;
+  uint32_t chunk = 8; /* tasks per work item */
+  uint32_t grid = 0;
+  std::vector<size_t> out_off;
+  size_t total_tasks = 0;
+  BatchJob *d_jobs = nullptr;
+  WorkItem *d_items = nullptr;
+  uint32_t n_items = 0;
+  uint32_t *d_counter = nullptr;
+  int *d_err = nullptr;
+  dto_record *d_out = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipStream_t own_stream = nullptr;
+};
+
+static void batch_free_plan(dto_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  if (b->d_jobs) (void)hipFree(b->d_jobs);
+  if (b->d_items) (void)hipFree(b->d_items);
+  b->d_jobs = nullptr;
+  b->d_items = nullptr;
+}
+
+/* Builds the job table and the work queue for the current shape and chunk. */
+static int batch_plan(dto_batch *b) {
+  batch_free_plan(b);
+  int rc = plan_shape(b->jobs, b->shape.nt, b->lds_limit, b->shape);
+  if (rc) return rc;
+  std::vector<BatchJob> hj(b->jobs.size());
+  for (size_t i = 0; i < b->jobs.size(); ++i) {
+    hj[i].J = b->jobs[i]->dev;
+    hj[i].lo = b->shape.lo[i];
+    hj[i].out = b->d_out + b->out_off[i];
+    hj[i].n_perm = (uint32_t)b->n_perm[i];
+    hj[i].job_id = (uint32_t)i;
+  }
+  /* largest jobs first (cost ~ cells x tasks) so the queue drains evenly */
+  std::vector<size_t> ord(b->jobs.size());
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+    double cx = (double)b->jobs[x]->dev.T1 * b->jobs[x]->dev.T2, cy = (double)b->jobs[y]->dev.T1 * b->jobs[y]->dev.T2;
+    return cx > cy;
+  });
+  std::vector<WorkItem> items;
+  for (size_t i : ord) {
+    uint64_t tasks = b->n_perm[i] + 1;
+    for (uint64_t t0 = 0; t0 < tasks; t0 += b->chunk) {
+      WorkItem w;
+      w.job = (uint32_t)i;
+      w.t0 = (uint32_t)t0;
+      w.count = (uint32_t)std::min<uint64_t>(b->chunk, tasks - t0);
+      w.pad = 0;
+      items.push_back(w);
+    }
+  }
+  b->n_items = (uint32_t)items.size();
+  HIP_TRY(hipMalloc(&b->d_jobs, hj.size() * sizeof(BatchJob)));
+  HIP_TRY(hipMalloc(&b->d_items, std::max<size_t>(1, items.size()) * sizeof(WorkItem)));
+  HIP_TRY(hipMemcpy(b->d_jobs, hj.data(), hj.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
+  if (!items.empty()) HIP_TRY(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  b->grid = grid_for(b->cus, b->shape.lds, b->shape.nt, b->n_items);
+  return DTO_OK;
+}
+
+extern "C" {
+
+int dto_batch_create(dto_job *const *jobs, const uint64_t *n_perm, uint64_t n_jobs, uint64_t seed, dto_batch **out) {
+  if (!out || (n_jobs && (!jobs || !n_perm))) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  *out = nullptr;
+  dto_batch *b = new dto_batch();
+  int rc = current_device(&b->device);
+  DevProps pr;
+  if (!rc) rc = device_props(b->device, &pr);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  b->cus = pr.cus;
+  b->lds_limit = pr.lds_per_block;
+  b->seed = seed;
+  for (uint64_t i = 0; i < n_jobs; ++i) {
+    if (!jobs[i] || n_perm[i] >= 0xFFFFFFFFull) {
+      delete b;
+      return fail(DTO_ERR_INVALID_ARGUMENT, "bad job or permutation count");
+    }
+    b->jobs.push_back(jobs[i]);
+    b->n_perm.push_back(n_perm[i]);
+    b->out_off.push_back(b->total_tasks);
+    b->total_tasks += n_perm[i] + 1;
+  }
+  if (hipMalloc(&b->d_out, std::max<size_t>(1, b->total_tasks) * sizeof(dto_record)) != hipSuccess ||
+      hipMalloc(&b->d_counter, 64) != hipSuccess || hipMalloc(&b->d_err, 64) != hipSuccess) {
+    dto_batch_destroy(b);
+    return fail(DTO_ERR_DEVICE, "hipMalloc failed for batch buffers");
+  }
+  b->shape.nt = 512;
+  if ((rc = batch_plan(b))) {
+    dto_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return DTO_OK;
+}
+
+int dto_batch_launch(dto_batch *b, void *stream) {
+  if (!b) return fail(DTO_ERR_INVALID_ARGUMENT, "null batch");
+  if (b->n_items == 0) return DTO_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (!st) {
+    if (!b->own_stream) HIP_TRY(hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking));
+    st = b->own_stream;
+  }
+  if (!b->exec) {
+    /* capture queue reset + persistent kernel once, replay afterwards */
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = launch_batch(b->shape.nt, b->shape.rmax, b->shape.hl, b->grid, b->shape.lds, st, b->d_jobs,
+                                b->d_items, b->n_items, b->seed, b->d_counter, b->d_err);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(DTO_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipGraphLaunch(b->exec, st));
+  if (!stream) HIP_TRY(hipStreamSynchronize(st));
+  return DTO_OK;
+}
+
+int dto_batch_results(dto_batch *b, dto_result *results) {
+  if (!b || !results) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  HIP_TRY(hipDeviceSynchronize());
+  int herr = 0;
+  HIP_TRY(hipMemcpy(&herr, b->d_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (herr) return fail(DTO_ERR_BAD_PERMUTATION, "permutation index out of range in batch");
+  std::vector<dto_record> rec(b->total_tasks);
+  HIP_TRY(hipMemcpy(rec.data(), b->d_out, b->total_tasks * sizeof(dto_record), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < b->jobs.size(); ++i) {
+    const dto_record *r = rec.data() + b->out_off[i];
+    int rc = dto_empirical_pvalue(&r[0], r + 1, b->n_perm[i], &results[i]);
+    if (rc) return rc;
+  }
+  return DTO_OK;
+}
+
+/* Times each candidate launch shape (threads per workgroup x tasks per work
+ * item) on this batch and keeps the fastest. */
+int dto_batch_autotune(dto_batch *b, int *waves) {
+  if (!b) return fail(DTO_ERR_INVALID_ARGUMENT, "null batch");
+  const int nts[2] = {256, 512};
+  const uint32_t chunks[3] = {1, 4, 16};
+  float best = 1e30f;
+  int best_nt = b->shape.nt;
+  uint32_t best_chunk = b->chunk;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  for (int nt : nts) {
+    bool ok = true;
+    for (dto_job *j : b->jobs)
+      if (rmax_for(j->dev.N2, nt) < 0) ok = false;
+    if (!ok) continue;
+    for (uint32_t c : chunks) {
+      b->shape.nt = nt;
+      b->chunk = c;
+      if (batch_plan(b)) continue;
+      float ms = 0.f;
+      bool good = true;
+      for (int rep = 0; rep < 2 && good; ++rep) {
+        HIP_TRY(hipEventRecord(e0, 0));
+        if (launch_batch(nt, b->shape.rmax, b->shape.hl, b->grid, b->shape.lds, 0, b->d_jobs, b->d_items, b->n_items,
+                         b->seed, b->d_counter, b->d_err) != hipSuccess)
+          good = false;
+        HIP_TRY(hipEventRecord(e1, 0));
+        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      }
+      if (good && ms < best) {
+        best = ms;
+        best_nt = nt;
+        best_chunk = c;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->shape.nt = best_nt;
+  b->chunk = best_chunk;
+  int rc = batch_plan(b);
+  if (waves) *waves = best_nt / 64;
+  return rc;
+}
+
+void dto_batch_destroy(dto_batch *b) {
+  if (!b) return;
+  batch_free_plan(b);
+  if (b->d_out) (void)hipFree(b->d_out);
+  if (b->d_counter) (void)hipFree(b->d_counter);
+  if (b->d_err) (void)hipFree(b->d_err);
+  if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
+  delete b;
+}
+
+}  // extern "C"
+
+/* ------------------------------------------------------------------ */
+/* server                                                                */
+/* ------------------------------------------------------------------ */
+namespace {
+
+struct Request {
+  uint64_t id = 0;
+  dto_job *job = nullptr;
+  uint64_t n_perm = 0;
+  uint64_t next_task = 0; /* next task to schedule (0 = unpermuted) */
+  uint64_t done_tasks = 0;
+  double t_submit = 0.0;
+  dto_record *d_out = nullptr;
+  size_t bytes = 0;
+  int rmax512 = 8;
+};
+
+struct Part {
+  Request *req;
+  uint32_t t0, count;
+};
+
+struct Slice {
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  BatchJob *d_jobs = nullptr;
+  WorkItem *d_items = nullptr;
+  BatchJob *h_jobs = nullptr; /* pinned staging */
+  WorkItem *h_items = nullptr;
+  uint32_t *d_counter = nullptr;
+  int *d_err = nullptr;
+  std::vector<Part> parts;
+  bool inflight = false;
+  uint32_t cap_jobs = 0, cap_items = 0;
+};
+
+}  // namespace
+
+struct dto_server {
+  dto_server_config cfg{};
+  int device = -1;
+  int cus = 0;
+  size_t lds_limit = 0;
+  uint64_t next_id = 1;
+  std::deque<Request *> queue;
+  std::vector<Request *> active;
+  size_t active_bytes = 0;
+  Slice slices[2];
+  uint32_t max_queue = 1u << 20;
+};
+
+static int slice_reserve(Slice &s, uint32_t jobs, uint32_t items) {
+  if (jobs > s.cap_jobs) {
+    if (s.d_jobs) (void)hipFree(s.d_jobs);
+    if (s.h_jobs) (void)hipHostFree(s.h_jobs);
+    HIP_TRY(hipMalloc(&s.d_jobs, jobs * sizeof(BatchJob)));
+    HIP_TRY(hipHostMalloc(&s.h_jobs, jobs * sizeof(BatchJob), hipHostMallocDefault));
+    s.cap_jobs = jobs;
+  }
+  if (items > s.cap_items) {
+    if (s.d_items) (void)hipFree(s.d_items);
+    if (s.h_items) (void)hipHostFree(s.h_items);
+    HIP_TRY(hipMalloc(&s.d_items, items * sizeof(WorkItem)));
+    HIP_TRY(hipHostMalloc(&s.h_items, items * sizeof(WorkItem), hipHostMallocDefault));
+    s.cap_items = items;
+  }
+  return DTO_OK;
+}
+
+/* Builds and launches one scheduling slice on slice s. */
+static int server_launch_slice(dto_server *sv, Slice &s) {
+  s.parts.clear();
+  for (Request *r : sv->active) {
+    uint64_t tasks = r->n_perm + 1;
+    if (r->next_task >= tasks) continue;
+    uint32_t c = (uint32_t)std::min<uint64_t>(sv->cfg.perms_per_slice, tasks - r->next_task);
+    s.parts.push_back(Part{r, (uint32_t)r->next_task, c});
+    r->next_task += c;
+  }
+  if (s.parts.empty()) return DTO_OK;
+  std::vector<dto_job *> jobs;
+  for (Part &p : s.parts) jobs.push_back(p.req->job);
+  Shape shape;
+  int rc = plan_shape(jobs, 512, sv->lds_limit, shape);
+  if (rc) return rc;
+  const uint32_t chunk = 4;
+  uint32_t n_items = 0;
+  for (Part &p : s.parts) n_items += (p.count + chunk - 1) / chunk;
+  if ((rc = slice_reserve(s, (uint32_t)s.parts.size(), n_items))) return rc;
+  uint32_t it = 0;
+  for (size_t i = 0; i < s.parts.size(); ++i) {
+    Part &p = s.parts[i];
+    BatchJob &bj = s.h_jobs[i];
+    bj.J = p.req->job->dev;
+    bj.lo = shape.lo[i];
+    bj.out = p.req->d_out;
+    bj.n_perm = (uint32_t)p.req->n_perm;
+    bj.job_id = (uint32_t)(p.req->id & 0xFFFFFFFFu);
+    for (uint32_t t0 = p.t0; t0 < p.t0 + p.count; t0 += chunk) {
+      WorkItem &w = s.h_items[it++];
+      w.job = (uint32_t)i;
+      w.t0 = t0;
+      w.count = std::min(chunk, p.t0 + p.count - t0);
+      w.pad = 0;
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(s.d_jobs, s.h_jobs, s.parts.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s.st));
+  HIP_TRY(hipMemcpyAsync(s.d_items, s.h_items, n_items * sizeof(WorkItem), hipMemcpyHostToDevice, s.st));
+  uint32_t grid = grid_for(sv->cus, shape.lds, 512, n_items);
+  hipError_t e = launch_batch(512, shape.rmax, shape.hl, grid, shape.lds, s.st, s.d_jobs, s.d_items, n_items,
+                              sv->cfg.seed, s.d_counter, s.d_err);
+  if (e != hipSuccess) return fail(DTO_ERR_DEVICE, std::string("slice launch: ") + hipGetErrorString(e));
+  HIP_TRY(hipEventRecord(s.ev, s.st));
+  s.inflight = true;
+  return DTO_OK;
+}
+
+extern "C" {
+
+int dto_server_create(const dto_server_config *cfg, dto_server **out) {
+  if (!cfg || !out) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  *out = nullptr;
+  dto_server *sv = new dto_server();
+  sv->cfg = *cfg;
+  if (!sv->cfg.max_active_jobs) sv->cfg.max_active_jobs = 256;
+  if (!sv->cfg.perms_per_slice) sv->cfg.perms_per_slice = 256;
+  if (!sv->cfg.hbm_budget_bytes) sv->cfg.hbm_budget_bytes = 8ull << 30;
+  DevProps pr;
+  int rc = current_device(&sv->device);
+  if (!rc) rc = device_props(sv->device, &pr);
+  if (rc) {
+    delete sv;
+    return rc;
+  }
+  sv->cus = pr.cus;
+  sv->lds_limit = pr.lds_per_block;
+  for (Slice &s : sv->slices) {
+    if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess || hipMalloc(&s.d_counter, 64) != hipSuccess ||
+        hipMalloc(&s.d_err, 64) != hipSuccess) {
+      dto_server_destroy(sv);
+      return fail(DTO_ERR_DEVICE, "server stream setup failed");
+    }
+  }
+  *out = sv;
+  return DTO_OK;
+}
+
+int dto_server_submit(dto_server *sv, dto_job *job, uint64_t n_perm, uint64_t *request_id) {
+  if (!sv || !job || !request_id) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  if (sv->queue.size() >= sv->max_queue) return fail(DTO_ERR_BUSY, "request queue full");
+  if (rmax_for(job->dev.N2, 512) < 0) return fail(DTO_ERR_TOO_LARGE, "list2 too long");
+  Request *r = new Request();
+  r->id = sv->next_id++;
+  r->job = job;
+  r->n_perm = n_perm;
+  r->t_submit = now_s();
+  r->bytes = job->arena_bytes + (n_perm + 1) * sizeof(dto_record);
+  sv->queue.push_back(r);
+  *request_id = r->id;
+  return DTO_OK;
+}
+
+int dto_server_pending(const dto_server *sv, uint64_t *queued, uint64_t *active) {
+  if (!sv) return fail(DTO_ERR_INVALID_ARGUMENT, "null server");
+  if (queued) *queued = sv->queue.size();
+  if (active) *active = sv->active.size();
+  return DTO_OK;
+}
+
+int dto_server_poll(dto_server *sv, uint32_t timeout_ms, uint64_t *ids, dto_result *results, double *latency_s,
+                    uint32_t cap, uint32_t *n_done) {
+  if (!sv || !n_done) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  *n_done = 0;
+  const double t_end = now_s() + timeout_ms * 1e-3;
+  for (;;) {
+    /* retire finished slices */
+    for (Slice &s : sv->slices) {
+      if (!s.inflight) continue;
+      hipError_t q = hipEventQuery(s.ev);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) return fail(DTO_ERR_DEVICE, std::string("slice: ") + hipGetErrorString(q));
+      int herr = 0;
+      HIP_TRY(hipMemcpy(&herr, s.d_err, sizeof(int), hipMemcpyDeviceToHost));
+      if (herr) return fail(DTO_ERR_BAD_PERMUTATION, "permutation index out of range in slice");
+      for (Part &p : s.parts) p.req->done_tasks += p.count;
+      s.parts.clear();
+      s.inflight = false;
+    }
+    /* complete requests */
+    for (size_t i = 0; i < sv->active.size() && *n_done < cap;) {
+      Request *r = sv->active[i];
+      bool busy = false;
+      for (Slice &s : sv->slices)
+        if (s.inflight)
+          for (Part &p : s.parts) busy |= p.req == r;
+      if (r->done_tasks == r->n_perm + 1 && !busy) {
+        std::vector<dto_record> rec(r->n_perm + 1);
+        HIP_TRY(hipMemcpy(rec.data(), r->d_out, rec.size() * sizeof(dto_record), hipMemcpyDeviceToHost));
+        int rc = dto_empirical_pvalue(&rec[0], rec.data() + 1, r->n_perm, &results[*n_done]);
+        if (rc) return rc;
+        if (ids) ids[*n_done] = r->id;
+        if (latency_s) latency_s[*n_done] = now_s() - r->t_submit;
+        ++*n_done;
+        (void)hipFree(r->d_out);
+        sv->active_bytes -= r->bytes;
+        dto_job_destroy(r->job);
+        delete r;
+        sv->active.erase(sv->active.begin() + (long)i);
+      } else {
+        ++i;
+      }
+    }
+    /* admit queued requests while their footprint fits */
+    while (!sv->queue.empty() && sv->active.size() < sv->cfg.max_active_jobs) {
+      Request *r = sv->queue.front();
+      if (!sv->active.empty() && sv->active_bytes + r->bytes > sv->cfg.hbm_budget_bytes) break;
+      HIP_TRY(hipMalloc(&r->d_out, (r->n_perm + 1) * sizeof(dto_record)));
+      sv->active_bytes += r->bytes;
+      sv->active.push_back(r);
+      sv->queue.pop_front();
+    }
+    /* keep both slices busy */
+    for (Slice &s : sv->slices)
+      if (!s.inflight) {
+        int rc = server_launch_slice(sv, s);
+        if (rc) return rc;
+      }
+    if (*n_done > 0 || now_s() >= t_end) return DTO_OK;
+    bool any = false;
+    for (Slice &s : sv->slices) any |= s.inflight;
+    if (!any && sv->queue.empty() && sv->active.empty()) return DTO_OK;
+    std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
+}
+
+void dto_server_destroy(dto_server *sv) {
+  if (!sv) return;
+  (void)hipDeviceSynchronize();
+  for (Slice &s : sv->slices) {
+    if (s.st) (void)hipStreamDestroy(s.st);
+    if (s.ev) (void)hipEventDestroy(s.ev);
+    if (s.d_jobs) (void)hipFree(s.d_jobs);
+    if (s.d_items) (void)hipFree(s.d_items);
+    if (s.h_jobs) (void)hipHostFree(s.h_jobs);
+    if (s.h_items) (void)hipHostFree(s.h_items);
+    if (s.d_counter) (void)hipFree(s.d_counter);
+    if (s.d_err) (void)hipFree(s.d_err);
+  }
+  for (Request *r : sv->active) {
+    if (r->d_out) (void)hipFree(r->d_out);
+    dto_job_destroy(r->job);
+    delete r;
+  }
+  for (Request *r : sv->queue) {
+    dto_job_destroy(r->job);
+    delete r;
+  }
+  delete sv;
+}
+
+}  // extern "C"

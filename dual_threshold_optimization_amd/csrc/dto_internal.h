@@ -1,0 +1,42 @@
+/*
+ * Library-internal state shared by the translation units of libdto_amd.so.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "dto_device.h"
+#include "dto_host.hpp"
+
+#define HIP_TRY(expr)                                                                                      \
+  do {                                                                                                     \
+    hipError_t e_ = (expr);                                                                                \
+    if (e_ != hipSuccess) return dto::fail(DTO_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace dto {
+constexpr int kNT = 512; /* threads per workgroup of the grid kernels (8 waves) */
+
+struct DevProps {
+  int ok = 0;
+  int cus = 0;
+  size_t lds_per_block = 0;
+};
+int current_device(int *dev);
+int device_props(int dev, DevProps *out);
+LdsLayout make_layout(const HostJob &h, int W, bool with_h);
+}  // namespace dto
+
+/* A job: host tables plus their device copy (one HBM arena). */
+struct dto_job {
+  dto::HostJob h;
+  dto::DevJob dev{};
+  void *arena = nullptr;
+  size_t arena_bytes = 0;
+  int *d_err = nullptr;
+  int device = -1;
+  dto::LdsLayout lo_screen{}, lo_plain{};
+  int rmax = 0;
+};
+int upload_job(dto_job *j);
