@@ -59,16 +59,14 @@ int dto::device_props(int dev, DevProps *out) {
 /* ------------------------------------------------------------------ */
 static uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 
-LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h) {
+LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limit) {
   LdsLayout L{};
   const uint32_t N1 = (uint32_t)h.l1.ids.size(), N2 = (uint32_t)h.l2.ids.size();
   const uint32_t T1 = (uint32_t)h.thr1.size(), T2 = (uint32_t)h.thr2.size();
   const uint32_t T2w = (T2 + 1) / 2;
   L.T2w_pad = (T2w + 3u) & ~3u;
   uint32_t o = 0;
-  L.h = o;
-  if (with_h) o = align16(o + 4u * (uint32_t)(h.N + 1));
-  L.h_in_lds = with_h;
+  /* fixed regions first */
   L.nb = o;
   o = align16(o + 2u * std::max(std::max(N1, N2), 1u));
   L.rowstart = o;
@@ -85,10 +83,27 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h) {
   o = align16(o + 4u * T2);
   L.cand = o;
   o = align16(o + 8u * (uint32_t)W * (uint32_t)kCandCap);
+  L.cand3 = o;
+  o = align16(o + 16u * (uint32_t)W * (uint32_t)kCand3Cap);
   L.exptab = o;
   o = align16(o + 8u * 256u);
   L.red = o;
   o = align16(o + 80u * (uint32_t)W);
+  /* screen table and pair staging: the pairs reuse the table's region */
+  L.h = L.pairs = o;
+  const uint32_t hb = with_h ? 4u * (uint32_t)(h.N + 1) : 0u, pb = 4u * std::max(N2, 1u);
+  if (with_h) {
+    L.h_in_lds = 1;
+    L.pairs_in_lds = 1;
+    o = align16(o + std::max(hb, pb));
+  } else if (o + pb <= lds_limit) {
+    L.h_in_lds = 0;
+    L.pairs_in_lds = 1;
+    o = align16(o + pb);
+  } else {
+    L.h_in_lds = 0;
+    L.pairs_in_lds = 0;
+  }
   L.total = o;
   return L;
 }
@@ -114,6 +129,7 @@ int upload_job(dto_job *j) {
   size_t o_t1 = put(buf, h.thr1), o_t2 = put(buf, h.thr2), o_LF = put(buf, h.LF), o_D = put(buf, h.D);
   size_t o_H = put(buf, h.H32), o_A = put(buf, h.A32), o_B = put(buf, h.B32);
   size_t o_err = put(buf, std::vector<int>(4, 0));
+  size_t o_stats = put(buf, std::vector<unsigned long long>(8, 0));
   if (j->arena) {
     (void)hipFree(j->arena);
     j->arena = nullptr;
@@ -147,44 +163,59 @@ int upload_job(dto_job *j) {
   d.A32 = (const float *)(b + o_A);
   d.B32 = (const float *)(b + o_B);
   j->d_err = (int *)(b + o_err);
+  d.stats = (unsigned long long *)(b + o_stats);
   /* LDS: keep the screen table resident when it fits */
-  LdsLayout with_h = make_layout(h, kNT / 64, true), without_h = make_layout(h, kNT / 64, false);
-  if (without_h.total > pr.lds_per_block)
+  const size_t lim = pr.lds_per_block;
+  LdsLayout with_h = make_layout(h, kNT / 64, true, lim), without_h = make_layout(h, kNT / 64, false, lim);
+  if (without_h.total > lim)
     return fail(DTO_ERR_TOO_LARGE, "job needs " + std::to_string(without_h.total) + " B of LDS per workgroup, device has " +
-                                       std::to_string(pr.lds_per_block));
+                                       std::to_string(lim));
   j->lo_plain = without_h;
-  j->lo_screen = with_h.total <= pr.lds_per_block ? with_h : without_h;
-  const uint32_t per = (d.N2 + kNT - 1) / kNT;
-  j->rmax = per <= 4 ? 4 : per <= 8 ? 8 : per <= 16 ? 16 : per <= 32 ? 32 : 64;
-  if (per > 64) return fail(DTO_ERR_TOO_LARGE, "list2 too long for the register-resident pair sort");
+  j->lo_screen = with_h.total <= lim ? with_h : without_h;
+  /* global pair staging for layouts whose pairs do not fit in LDS */
+  if (j->ws) {
+    (void)hipFree(j->ws);
+    j->ws = nullptr;
+  }
+  if (!j->lo_plain.pairs_in_lds || !j->lo_screen.pairs_in_lds) {
+    j->ws_slots = (uint32_t)pr.cus * 4u;
+    HIP_TRY(hipMalloc(&j->ws, (size_t)j->ws_slots * std::max<uint32_t>(d.N2, 1) * 4));
+  }
   return DTO_OK;
 }
 
 /* ------------------------------------------------------------------ */
 /* launches                                                              */
 /* ------------------------------------------------------------------ */
-template <int R, bool E, bool HL, bool PL>
+template <bool E, bool HL, bool PL, bool PA, bool RC>
 static hipError_t launch_opt(const dto_job *j, const LdsLayout &lo, uint32_t grid, hipStream_t st,
                              const uint32_t *i1, const uint32_t *i2, uint64_t n_perm, uint64_t base, uint64_t seed,
                              uint32_t job_id, int32_t permuted, dto_record *out) {
-  auto kern = k_optimize<kNT, R, E, HL, PL>;
+  auto kern = k_optimize<kNT, E, HL, PL, PA, RC>;
   (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lo.total);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kNT), lo.total, st, j->dev, lo, i1, i2, n_perm, base, seed, job_id,
-                     permuted, out, j->d_err);
+                     permuted, out, j->d_err, j->ws);
   return hipGetLastError();
 }
+
+/* Lane segments fit in registers when a row is at most kSegWords words per lane. */
+static bool regcols(const dto_job *j) { return (j->dev.T2w + 63) / 64 <= kSegWords; }
 
 template <bool E, bool HL, bool PL>
 static hipError_t launch_opt_r(const dto_job *j, const LdsLayout &lo, uint32_t grid, hipStream_t st,
                                const uint32_t *i1, const uint32_t *i2, uint64_t n_perm, uint64_t base,
                                uint64_t seed, uint32_t job_id, int32_t permuted, dto_record *out) {
-  switch (j->rmax) {
-    case 4: return launch_opt<4, E, HL, PL>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
-    case 8: return launch_opt<8, E, HL, PL>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
-    case 16: return launch_opt<16, E, HL, PL>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
-    case 32: return launch_opt<32, E, HL, PL>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
-    default: return launch_opt<64, E, HL, PL>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
+  if (PL) /* the plain kernel reads column constants per cell; one variant */
+    return lo.pairs_in_lds
+               ? launch_opt<E, false, true, true, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out)
+               : launch_opt<E, false, true, false, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
+  const bool rc = regcols(j);
+  if (lo.pairs_in_lds) {
+    if (rc) return launch_opt<E, HL, false, true, true>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
+    return launch_opt<E, HL, false, true, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
   }
+  if (rc) return launch_opt<E, false, false, false, true>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
+  return launch_opt<E, false, false, false, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
 }
 
 /* Number of workgroups for a permutation launch: enough to fill every CU at
@@ -195,6 +226,7 @@ static uint32_t perm_grid(const dto_job *j, const LdsLayout &lo, uint64_t n_perm
   uint64_t per_cu = std::max<uint64_t>(1, pr.lds_per_block ? (160u * 1024u) / std::max<uint32_t>(lo.total, 1) : 1);
   per_cu = std::min<uint64_t>(per_cu, 4);
   uint64_t g = std::min<uint64_t>(n_perm, (uint64_t)pr.cus * per_cu);
+  if (!lo.pairs_in_lds) g = std::min<uint64_t>(g, j->ws_slots); /* one staging slot per workgroup */
   return (uint32_t)std::max<uint64_t>(g, 1);
 }
 
@@ -365,6 +397,16 @@ int dto_internal_shuffle_device(uint64_t seed, uint32_t job, uint32_t perm, uint
   return DTO_OK;
 }
 
+/* work counters of a job's screened launches: exact cells, exact terms,
+ * rescanned rows (test / profiling hook, not in dto.h) */
+int dto_internal_stats(dto_job *job, uint64_t *out, int reset) {
+  if (!job || !out) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(out, job->dev.stats, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if (reset) HIP_TRY(hipMemset(job->dev.stats, 0, 8 * sizeof(uint64_t)));
+  return DTO_OK;
+}
+
 int dto_philox_permutation(uint64_t seed, uint32_t job_id, uint32_t perm, uint32_t list, uint32_t len, uint32_t *out) {
   if (!out && len) return fail(DTO_ERR_INVALID_ARGUMENT, "null output");
   Shuffle s = shuffle_make(seed, job_id, perm, list, len);
@@ -461,6 +503,7 @@ int dto_job_thresholds(const dto_job *job, int list, uint32_t *out, uint32_t cap
 void dto_job_destroy(dto_job *job) {
   if (!job) return;
   if (job->arena) (void)hipFree(job->arena);
+  if (job->ws) (void)hipFree(job->ws);
   delete job;
 }
 
@@ -513,20 +556,16 @@ int dto_process_threshold_pairs(dto_job *job, const uint32_t *idx1, const uint32
     HIP_TRY(hipMemcpy(d2, idx2, n2 * 4, hipMemcpyHostToDevice));
   }
   const LdsLayout &lo = job->lo_plain;
-  switch (job->rmax) {
-#define DTO_GRID_CASE(R)                                                                                           \
-  case R:                                                                                                          \
-    (void)hipFuncSetAttribute((const void *)k_grid_counts<kNT, R>, hipFuncAttributeMaxDynamicSharedMemorySize,     \
-                              (int)lo.total);                                                                      \
-    hipLaunchKernelGGL((k_grid_counts<kNT, R>), dim3(1), dim3(kNT), lo.total, 0, job->dev, lo, d1, d2,             \
-                       (uint32_t *)dk.p, job->d_err);                                                              \
-    break;
-    DTO_GRID_CASE(4)
-    DTO_GRID_CASE(8)
-    DTO_GRID_CASE(16)
-    DTO_GRID_CASE(32)
-    DTO_GRID_CASE(64)
-#undef DTO_GRID_CASE
+  if (lo.pairs_in_lds) {
+    (void)hipFuncSetAttribute((const void *)k_grid_counts<kNT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lo.total);
+    hipLaunchKernelGGL((k_grid_counts<kNT, true>), dim3(1), dim3(kNT), lo.total, 0, job->dev, lo, d1, d2,
+                       (uint32_t *)dk.p, job->d_err, job->ws);
+  } else {
+    (void)hipFuncSetAttribute((const void *)k_grid_counts<kNT, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lo.total);
+    hipLaunchKernelGGL((k_grid_counts<kNT, false>), dim3(1), dim3(kNT), lo.total, 0, job->dev, lo, d1, d2,
+                       (uint32_t *)dk.p, job->d_err, job->ws);
   }
   HIP_TRY(hipGetLastError());
   int32_t permuted = (idx1 || idx2) ? 1 : 0;

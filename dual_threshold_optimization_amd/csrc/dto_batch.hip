@@ -33,73 +33,76 @@ using namespace dto;
 
 namespace {
 
-/* Launch shapes the batch kernels are built for. */
-template <int NT, int R, bool HL>
+/* Launch shapes the batch kernels are built for: threads per workgroup, the
+ * screen table in LDS or not, the pair staging in LDS or global memory. */
+template <int NT, bool HL, bool PL, bool RC>
 hipError_t launch_batch_t(uint32_t grid, uint32_t lds, hipStream_t st, const BatchJob *jobs, const WorkItem *items,
-                          uint32_t n_items, uint64_t seed, uint32_t *counter, int *err) {
-  auto kern = k_batch<NT, R, HL>;
+                          uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, uint32_t *ws,
+                          uint32_t ws_stride) {
+  auto kern = k_batch<NT, HL, PL, RC>;
   (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, jobs, items, n_items, seed, counter, err);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride);
   return hipGetLastError();
 }
 
-template <int NT, bool HL>
-hipError_t launch_batch_r(int rmax, uint32_t grid, uint32_t lds, hipStream_t st, const BatchJob *jobs,
-                          const WorkItem *items, uint32_t n_items, uint64_t seed, uint32_t *counter, int *err) {
-  switch (rmax) {
-    case 8: return launch_batch_t<NT, 8, HL>(grid, lds, st, jobs, items, n_items, seed, counter, err);
-    case 16: return launch_batch_t<NT, 16, HL>(grid, lds, st, jobs, items, n_items, seed, counter, err);
-    case 32: return launch_batch_t<NT, 32, HL>(grid, lds, st, jobs, items, n_items, seed, counter, err);
-    default: return launch_batch_t<NT, 64, HL>(grid, lds, st, jobs, items, n_items, seed, counter, err);
-  }
-}
-
-hipError_t launch_batch(int nt, int rmax, bool hl, uint32_t grid, uint32_t lds, hipStream_t st, const BatchJob *jobs,
-                        const WorkItem *items, uint32_t n_items, uint64_t seed, uint32_t *counter, int *err) {
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, counter, err);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  if (nt == 256)
-    return hl ? launch_batch_r<256, true>(rmax, grid, lds, st, jobs, items, n_items, seed, counter, err)
-              : launch_batch_r<256, false>(rmax, grid, lds, st, jobs, items, n_items, seed, counter, err);
-  return hl ? launch_batch_r<512, true>(rmax, grid, lds, st, jobs, items, n_items, seed, counter, err)
-            : launch_batch_r<512, false>(rmax, grid, lds, st, jobs, items, n_items, seed, counter, err);
-}
-
-/* Pair-register count for a list2 length at nt threads. */
-int rmax_for(uint32_t n2, int nt) {
-  uint32_t per = (n2 + nt - 1) / nt;
-  return per <= 8 ? 8 : per <= 16 ? 16 : per <= 32 ? 32 : per <= 64 ? 64 : -1;
+template <int NT>
+hipError_t launch_batch_n(bool hl, bool pl, bool rc, uint32_t grid, uint32_t lds, hipStream_t st,
+                          const BatchJob *jobs, const WorkItem *items, uint32_t n_items, uint64_t seed,
+                          uint32_t *counter, int *err, uint32_t *ws, uint32_t ws_stride) {
+  if (hl)
+    return rc ? launch_batch_t<NT, true, true, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride)
+              : launch_batch_t<NT, true, true, false>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride);
+  if (pl)
+    return rc ? launch_batch_t<NT, false, true, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride)
+              : launch_batch_t<NT, false, true, false>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride);
+  return rc ? launch_batch_t<NT, false, false, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride)
+            : launch_batch_t<NT, false, false, false>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride);
 }
 
 /* Batch-wide launch parameters for a set of jobs at nt threads. */
 struct Shape {
   int nt = 512;
-  int rmax = 8;
   bool hl = true;
+  bool pl = true;
+  bool rc = true; /* every job's lane segment fits in registers */
   uint32_t lds = 0;
+  uint32_t max_n2 = 0;
   std::vector<LdsLayout> lo;
 };
 
+hipError_t launch_batch(const Shape &sh, uint32_t grid, hipStream_t st, const BatchJob *jobs, const WorkItem *items,
+                        uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, uint32_t *ws) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, counter, err);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (sh.nt == 256)
+    return launch_batch_n<256>(sh.hl, sh.pl, sh.rc, grid, sh.lds, st, jobs, items, n_items, seed, counter, err, ws,
+                               sh.max_n2);
+  return launch_batch_n<512>(sh.hl, sh.pl, sh.rc, grid, sh.lds, st, jobs, items, n_items, seed, counter, err, ws,
+                             sh.max_n2);
+}
+
 int plan_shape(const std::vector<dto_job *> &jobs, int nt, size_t lds_limit, Shape &s) {
   s.nt = nt;
-  s.rmax = 8;
-  bool fit_h = true;
+  bool fit_h = true, fit_p = true;
+  s.max_n2 = 1;
+  s.rc = true;
   for (dto_job *j : jobs) {
-    int r = rmax_for(j->dev.N2, nt);
-    if (r < 0) return fail(DTO_ERR_TOO_LARGE, "list2 too long for this launch shape");
-    s.rmax = std::max(s.rmax, r);
-    if (make_layout(j->h, nt / 64, true).total > lds_limit) fit_h = false;
+    if ((j->dev.T2w + 63) / 64 > kSegWords) s.rc = false;
+    if (make_layout(j->h, nt / 64, true, lds_limit).total > lds_limit) fit_h = false;
+    s.max_n2 = std::max(s.max_n2, j->dev.N2);
   }
-  s.hl = fit_h;
   s.lo.clear();
   s.lds = 0;
   for (dto_job *j : jobs) {
-    LdsLayout L = make_layout(j->h, nt / 64, fit_h);
+    LdsLayout L = make_layout(j->h, nt / 64, fit_h, lds_limit);
     if (L.total > lds_limit) return fail(DTO_ERR_TOO_LARGE, "job does not fit in LDS");
+    if (!L.pairs_in_lds) fit_p = false;
     s.lo.push_back(L);
     s.lds = std::max(s.lds, L.total);
   }
+  s.hl = fit_h;
+  s.pl = fit_p;
   return DTO_OK;
 }
 
@@ -138,6 +141,7 @@ struct dto_batch {
   uint32_t *d_counter = nullptr;
   int *d_err = nullptr;
   dto_record *d_out = nullptr;
+  uint32_t *d_ws = nullptr; /* global pair staging when the LDS cannot hold the pairs */
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t own_stream = nullptr;
@@ -150,8 +154,10 @@ static void batch_free_plan(dto_batch *b) {
   b->graph = nullptr;
   if (b->d_jobs) (void)hipFree(b->d_jobs);
   if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_ws) (void)hipFree(b->d_ws);
   b->d_jobs = nullptr;
   b->d_items = nullptr;
+  b->d_ws = nullptr;
 }
 
 /* Builds the job table and the work queue for the current shape and chunk. */
@@ -192,6 +198,7 @@ static int batch_plan(dto_batch *b) {
   HIP_TRY(hipMemcpy(b->d_jobs, hj.data(), hj.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
   if (!items.empty()) HIP_TRY(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
   b->grid = grid_for(b->cus, b->shape.lds, b->shape.nt, b->n_items);
+  if (!b->shape.pl) HIP_TRY(hipMalloc(&b->d_ws, (size_t)b->grid * b->shape.max_n2 * 4));
   return DTO_OK;
 }
 
@@ -246,8 +253,8 @@ int dto_batch_launch(dto_batch *b, void *stream) {
   if (!b->exec) {
     /* capture queue reset + persistent kernel once, replay afterwards */
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t e = launch_batch(b->shape.nt, b->shape.rmax, b->shape.hl, b->grid, b->shape.lds, st, b->d_jobs,
-                                b->d_items, b->n_items, b->seed, b->d_counter, b->d_err);
+    hipError_t e = launch_batch(b->shape, b->grid, st, b->d_jobs, b->d_items, b->n_items, b->seed, b->d_counter,
+                                b->d_err, b->d_ws);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(st, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -291,10 +298,6 @@ int dto_batch_autotune(dto_batch *b, int *waves) {
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   for (int nt : nts) {
-    bool ok = true;
-    for (dto_job *j : b->jobs)
-      if (rmax_for(j->dev.N2, nt) < 0) ok = false;
-    if (!ok) continue;
     for (uint32_t c : chunks) {
       b->shape.nt = nt;
       b->chunk = c;
@@ -303,8 +306,8 @@ int dto_batch_autotune(dto_batch *b, int *waves) {
       bool good = true;
       for (int rep = 0; rep < 2 && good; ++rep) {
         HIP_TRY(hipEventRecord(e0, 0));
-        if (launch_batch(nt, b->shape.rmax, b->shape.hl, b->grid, b->shape.lds, 0, b->d_jobs, b->d_items, b->n_items,
-                         b->seed, b->d_counter, b->d_err) != hipSuccess)
+        if (launch_batch(b->shape, b->grid, 0, b->d_jobs, b->d_items, b->n_items, b->seed, b->d_counter, b->d_err,
+                         b->d_ws) != hipSuccess)
           good = false;
         HIP_TRY(hipEventRecord(e1, 0));
         HIP_TRY(hipEventSynchronize(e1));
@@ -352,7 +355,6 @@ struct Request {
   double t_submit = 0.0;
   dto_record *d_out = nullptr;
   size_t bytes = 0;
-  int rmax512 = 8;
 };
 
 struct Part {
@@ -369,6 +371,8 @@ struct Slice {
   WorkItem *h_items = nullptr;
   uint32_t *d_counter = nullptr;
   int *d_err = nullptr;
+  uint32_t *d_ws = nullptr;
+  size_t ws_words = 0;
   std::vector<Part> parts;
   bool inflight = false;
   uint32_t cap_jobs = 0, cap_items = 0;
@@ -447,8 +451,13 @@ static int server_launch_slice(dto_server *sv, Slice &s) {
   HIP_TRY(hipMemcpyAsync(s.d_jobs, s.h_jobs, s.parts.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s.st));
   HIP_TRY(hipMemcpyAsync(s.d_items, s.h_items, n_items * sizeof(WorkItem), hipMemcpyHostToDevice, s.st));
   uint32_t grid = grid_for(sv->cus, shape.lds, 512, n_items);
-  hipError_t e = launch_batch(512, shape.rmax, shape.hl, grid, shape.lds, s.st, s.d_jobs, s.d_items, n_items,
-                              sv->cfg.seed, s.d_counter, s.d_err);
+  if (!shape.pl && (size_t)grid * shape.max_n2 > s.ws_words) {
+    if (s.d_ws) (void)hipFree(s.d_ws);
+    s.ws_words = (size_t)grid * shape.max_n2;
+    HIP_TRY(hipMalloc(&s.d_ws, s.ws_words * 4));
+  }
+  hipError_t e = launch_batch(shape, grid, s.st, s.d_jobs, s.d_items, n_items, sv->cfg.seed, s.d_counter, s.d_err,
+                              s.d_ws);
   if (e != hipSuccess) return fail(DTO_ERR_DEVICE, std::string("slice launch: ") + hipGetErrorString(e));
   HIP_TRY(hipEventRecord(s.ev, s.st));
   s.inflight = true;
@@ -489,7 +498,6 @@ int dto_server_create(const dto_server_config *cfg, dto_server **out) {
 int dto_server_submit(dto_server *sv, dto_job *job, uint64_t n_perm, uint64_t *request_id) {
   if (!sv || !job || !request_id) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
   if (sv->queue.size() >= sv->max_queue) return fail(DTO_ERR_BUSY, "request queue full");
-  if (rmax_for(job->dev.N2, 512) < 0) return fail(DTO_ERR_TOO_LARGE, "list2 too long");
   Request *r = new Request();
   r->id = sv->next_id++;
   r->job = job;
@@ -586,6 +594,7 @@ void dto_server_destroy(dto_server *sv) {
     if (s.h_items) (void)hipHostFree(s.h_items);
     if (s.d_counter) (void)hipFree(s.d_counter);
     if (s.d_err) (void)hipFree(s.d_err);
+    if (s.d_ws) (void)hipFree(s.d_ws);
   }
   for (Request *r : sv->active) {
     if (r->d_out) (void)hipFree(r->d_out);

@@ -35,11 +35,14 @@ struct DevJob {
   const float *H32;         /* [N+1] LF[x] - x*(ln N - 1), rounded to f32 */
   const float *A32;         /* [T1] H(K_i) + H(N-K_i) */
   const float *B32;         /* [T2] H(n_j) + H(N-n_j) - H(N) */
+  unsigned long long *stats; /* [8] work counters: 0 exact cells, 1 exact terms, 2 rows rescanned */
 };
 
 /* Byte offsets of the dynamic LDS regions of one workgroup. */
 struct LdsLayout {
   uint32_t h;        /* f32[N+1] screen table (only when h_in_lds) */
+  uint32_t pairs;    /* u32[N2] (a,b) staging: aliases h when h_in_lds */
+  uint32_t pairs_in_lds;
   uint32_t nb;       /* u16[max(N1,N2)]: list1 bins after the shuffle, then row-sorted list2 bins */
   uint32_t rowstart; /* u32[T1+1] */
   uint32_t cursor;   /* u32[T1] */
@@ -47,7 +50,8 @@ struct LdsLayout {
   uint32_t rowmin;   /* f32[T1] per-row minimum screen lower bound */
   uint32_t coln;     /* u32[T2] */
   uint32_t colb;     /* f32[T2] */
-  uint32_t cand;     /* uint2[W][kCandCap] */
+  uint32_t cand;     /* uint2[W][kCandCap] tier-1 survivors */
+  uint32_t cand3;    /* uint4[W][kCand3Cap] tier-2 survivors */
   uint32_t exptab;   /* u64[256] */
   uint32_t red;      /* per-wave reduction slots */
   uint32_t total;
@@ -55,7 +59,8 @@ struct LdsLayout {
   uint32_t h_in_lds;
 };
 
-constexpr int kCandCap = 128;
+constexpr int kCandCap = 64;
+constexpr int kCand3Cap = 32;
 
 /* The best cell found so far: key (p, -k, i, j), lexicographic minimum. */
 struct BestKey {

@@ -25,7 +25,7 @@ struct DevProps {
 };
 int current_device(int *dev);
 int device_props(int dev, DevProps *out);
-LdsLayout make_layout(const HostJob &h, int W, bool with_h);
+LdsLayout make_layout(const HostJob &h, int W, bool with_h, size_t lds_limit);
 }  // namespace dto
 
 /* A job: host tables plus their device copy (one HBM arena). */
@@ -37,6 +37,7 @@ struct dto_job {
   int *d_err = nullptr;
   int device = -1;
   dto::LdsLayout lo_screen{}, lo_plain{};
-  int rmax = 0;
+  uint32_t *ws = nullptr; /* global pair staging (only when the LDS cannot hold the pairs) */
+  uint32_t ws_slots = 0;
 };
 int upload_job(dto_job *j);

@@ -70,7 +70,7 @@ __device__ __forceinline__ void take_better(BestKey &b, double p, uint32_t k, ui
   }
 }
 
-__device__ BestKey wave_best(BestKey b) {
+__device__ __forceinline__ BestKey wave_best(BestKey b) {
   for (int off = 32; off >= 1; off >>= 1) {
     double p = __shfl_xor(b.p, off);
     uint32_t k = __shfl_xor(b.k, off);
@@ -80,24 +80,36 @@ __device__ BestKey wave_best(BestKey b) {
   return b;
 }
 
-__device__ __forceinline__ float wave_minf(float v) {
-  for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off));
-  return v;
+/* Wave-wide scans with DPP (gfx9 family): row_shr 1, 2, 4, 8 within each
+ * 16-lane row, then row_bcast:15 and row_bcast:31 carry the row totals. */
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false); /* row_shr:1 */
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false); /* row_shr:2 */
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false); /* row_shr:4 */
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false); /* row_shr:8 */
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); /* row_bcast:15 */
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); /* row_bcast:31 */
+  return x;
 }
 
-__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
-  const uint32_t lane = lane_id();
-  uint32_t x = v;
-  for (uint32_t off = 1; off < 64; off <<= 1) {
-    uint32_t y = __shfl_up(x, off);
-    if (lane >= off) x += y;
-  }
-  return x - v;
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) { return wave_incl_scan(v) - v; }
+
+/* Minimum over the wave, returned in every lane. */
+__device__ __forceinline__ float wave_minf(float v) {
+  const int inf = __float_as_int(__builtin_inff());
+  int x = __float_as_int(v);
+  x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(inf, x, 0x111, 0xf, 0xf, false))));
+  x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(inf, x, 0x112, 0xf, 0xf, false))));
+  x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(inf, x, 0x114, 0xf, 0xf, false))));
+  x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(inf, x, 0x118, 0xf, 0xf, false))));
+  x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(inf, x, 0x142, 0xa, 0xf, false))));
+  x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(inf, x, 0x143, 0xc, 0xf, false))));
+  return __int_as_float(__builtin_amdgcn_readlane(x, 63));
 }
 
 /* Exclusive scan of data[0..len) in LDS, total written to data[len]. */
 template <int NT>
-__device__ void block_excl_scan(uint32_t *data, uint32_t len, uint32_t *wave_sums) {
+__device__ __forceinline__ void block_excl_scan(uint32_t *data, uint32_t len, uint32_t *wave_sums) {
   const uint32_t tid = threadIdx.x, w = tid >> 6;
   const uint32_t per = (len + NT - 1) / NT;
   const uint32_t beg = min(tid * per, len), end = min(beg + per, len);
@@ -119,21 +131,25 @@ __device__ void block_excl_scan(uint32_t *data, uint32_t len, uint32_t *wave_sum
 
 struct Lds {
   float *h;
+  uint32_t *pairs;
   uint16_t *nb;
   uint32_t *rowstart, *cursor, *cw;
   float *rowmin;
   uint32_t *coln;
   float *colb;
   uint2 *cand;
+  uint4 *cand3;
   uint64_t *tab;
   float *redU;
   BestKey *redB; /* [3][W]: zero class, one class, evaluated */
   uint32_t *redS;
+  float *redU2;
 };
 
 __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo, int W) {
   Lds s;
   s.h = (float *)(smem + Lo.h);
+  s.pairs = (uint32_t *)(smem + Lo.pairs);
   s.nb = (uint16_t *)(smem + Lo.nb);
   s.rowstart = (uint32_t *)(smem + Lo.rowstart);
   s.cursor = (uint32_t *)(smem + Lo.cursor);
@@ -142,105 +158,166 @@ __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo
   s.coln = (uint32_t *)(smem + Lo.coln);
   s.colb = (float *)(smem + Lo.colb);
   s.cand = (uint2 *)(smem + Lo.cand);
+  s.cand3 = (uint4 *)(smem + Lo.cand3);
   s.tab = (uint64_t *)(smem + Lo.exptab);
   s.redU = (float *)(smem + Lo.red);
   s.redB = (BestKey *)(smem + Lo.red + 16 * W);
   s.redS = (uint32_t *)(smem + Lo.red + 16 * W + 3 * W * sizeof(BestKey));
+  s.redU2 = (float *)(smem + Lo.red + 16 * W + 3 * W * sizeof(BestKey) + 4 * W);
   return s;
+}
+
+/* Copies n floats global -> LDS with 16-byte loads (both 16-byte aligned). */
+template <int NT>
+__device__ __forceinline__ void copy_table(float *dst, const float *src, uint32_t n) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t n4 = n / 4;
+  const float4 *s4 = reinterpret_cast<const float4 *>(src);
+  float4 *d4 = reinterpret_cast<float4 *>(dst);
+  for (uint32_t t0 = tid; t0 < n4; t0 += NT * 4) {
+    float4 v[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u)
+      if (t0 + u * NT < n4) v[u] = s4[t0 + u * NT];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u)
+      if (t0 + u * NT < n4) d4[t0 + u * NT] = v[u];
+  }
+  for (uint32_t t = n4 * 4 + tid; t < n; t += NT) dst[t] = src[t];
 }
 
 /* Per-job tables that stay in LDS while a workgroup works on that job. */
 template <int NT, bool HLDS>
-__device__ void load_job_tables(const DevJob &J, const Lds &S) {
+__device__ __forceinline__ void load_job_tables(const DevJob &J, const Lds &S) {
   const uint32_t tid = threadIdx.x;
   for (uint32_t t = tid; t < 256; t += NT) S.tab[t] = kExpTabC[t];
-  if (HLDS)
-    for (uint32_t t = tid; t <= J.N; t += NT) S.h[t] = J.H32[t];
+  if (HLDS) copy_table<NT>(S.h, J.H32, J.N + 1);
   for (uint32_t t = tid; t < J.T2; t += NT) {
     S.coln[t] = J.n[t];
     S.colb[t] = J.B32[t];
   }
 }
 
-/* Phases A-C for one permutation: leaves the row-sorted list2 bins in S.nb,
- * row offsets in S.rowstart, and in each wave's S.cw block the column counts
- * of all pairs in rows before the wave's first row. */
-template <int NT, int RMAX, bool EXPLICIT>
-__device__ void build_rows(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
-                           const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err) {
+/* Phases A-B for one permutation: leaves the row-sorted list2 bins in S.nb,
+ * row offsets in S.rowstart and zeroed per-wave column counts.  The (a, b) pairs are
+ * staged in `pairs` (LDS: the screen-table region, reloaded afterwards when
+ * HLDS; or a per-workgroup global buffer when the LDS cannot hold them). */
+template <int NT, bool EXPLICIT, bool HLDS, bool PL>
+__device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
+                                           const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
+                                           uint32_t *gpairs) {
   constexpr int W = NT / 64;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
   const uint32_t T1 = J.T1, T2 = J.T2, T2wp = Lo.T2w_pad;
+  uint32_t *lpairs = S.pairs;
+  volatile uint32_t *vpairs = gpairs; /* global staging: bypass the per-CU cache on the way back */
   for (uint32_t t = tid; t <= T1; t += NT) S.rowstart[t] = 0;
   for (uint32_t t = tid; t < W * T2wp; t += NT) S.cw[t] = 0;
-  /* A: list1 feature m sits at the position k with idx1[k] == m */
-  for (uint32_t k = tid; k < J.N1; k += NT) {
-    uint32_t m = EXPLICIT ? (i1 ? i1[k] : k) : shuffle_index(s1, k);
-    if (m < J.N1)
-      S.nb[m] = J.bin1[k];
-    else
-      atomicOr(err, 1);
+  /* A: list1 feature m sits at the position k with idx1[k] == m (loads of
+   * kU positions in flight per thread) */
+  constexpr uint32_t kU = 8;
+  for (uint32_t k0 = tid; k0 < J.N1; k0 += NT * kU) {
+    uint32_t m[kU], bn[kU];
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      const uint32_t k = k0 + u * NT;
+      m[u] = 0u;
+      bn[u] = 0u;
+      if (k < J.N1) {
+        m[u] = EXPLICIT ? (i1 ? i1[k] : k) : shuffle_index(s1, k);
+        bn[u] = J.bin1[k];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      if (k0 + u * NT < J.N1) {
+        if (m[u] < J.N1)
+          S.nb[m[u]] = (uint16_t)bn[u];
+        else
+          atomicOr(err, 1);
+      }
+    }
   }
   __syncthreads();
   /* B1: (a, b) per list2 position, counted per row a */
-  uint32_t pr[RMAX];
+  for (uint32_t e0 = tid; e0 < J.N2; e0 += NT * kU) {
+    uint32_t q[kU], b[kU];
+    int32_t jm[kU];
 #pragma unroll
-  for (int r = 0; r < RMAX; ++r) {
-    const uint32_t e = tid + (uint32_t)r * NT;
-    pr[r] = kNone;
-    if (e < J.N2) {
-      uint32_t q = EXPLICIT ? (i2 ? i2[e] : e) : shuffle_index(s2, e);
+    for (uint32_t u = 0; u < kU; ++u) {
+      const uint32_t e = e0 + u * NT;
+      q[u] = kNone;
+      b[u] = kBinNone;
+      if (e < J.N2) {
+        q[u] = EXPLICIT ? (i2 ? i2[e] : e) : shuffle_index(s2, e);
+        b[u] = J.bin2[e];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) jm[u] = (q[u] < J.N2 && !J.dup) ? J.join[q[u]] : -1;
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      const uint32_t e = e0 + u * NT;
+      if (e >= J.N2) continue;
       uint32_t a = kBinNone;
-      if (q >= J.N2) {
+      if (q[u] >= J.N2) {
         atomicOr(err, 1);
       } else if (!J.dup) {
-        int32_t m = J.join[q];
-        if (m >= 0) a = S.nb[m];
+        if (jm[u] >= 0) a = S.nb[jm[u]];
       } else {
         /* intersect_genes collapses duplicate ids of genes1: the entry is in
          * S1'(t) as soon as any list1 entry with its id is */
-        for (uint32_t t = J.join_off[q]; t < J.join_off[q + 1]; ++t) {
+        for (uint32_t t = J.join_off[q[u]]; t < J.join_off[q[u] + 1]; ++t) {
           uint32_t v = S.nb[J.join_pos[t]];
           a = v < a ? v : a;
         }
       }
-      uint32_t b = J.bin2[e];
-      if (a < T1 && b < T2) {
-        pr[r] = (a << 16) | b;
+      uint32_t pr = kNone;
+      if (a < T1 && b[u] < T2) {
+        pr = (a << 16) | b[u];
         atomicAdd(&S.rowstart[a], 1u);
       }
+      if (PL)
+        lpairs[e] = pr;
+      else
+        vpairs[e] = pr;
     }
   }
+  if (!PL) __threadfence_block();
   __syncthreads();
   block_excl_scan<NT>(S.rowstart, T1, S.redS);
   for (uint32_t t = tid; t < T1; t += NT) S.cursor[t] = S.rowstart[t];
   __syncthreads();
   /* B2: scatter b into row order (overwrites the list1 bins, all read above) */
-#pragma unroll
-  for (int r = 0; r < RMAX; ++r) {
-    if (pr[r] != kNone) {
-      uint32_t pos = atomicAdd(&S.cursor[pr[r] >> 16], 1u);
-      S.nb[pos] = (uint16_t)(pr[r] & 0xFFFFu);
+  for (uint32_t e = tid; e < J.N2; e += NT) {
+    const uint32_t pr = PL ? lpairs[e] : vpairs[e];
+    if (pr != kNone) {
+      uint32_t pos = atomicAdd(&S.cursor[pr >> 16], 1u);
+      S.nb[pos] = (uint16_t)(pr & 0xFFFFu);
     }
   }
   __syncthreads();
-  /* C: per-wave column counts of its own row block, then exclusive prefix over waves */
-  const uint32_t r0 = (uint32_t)(((uint64_t)w * T1) / W), r1 = (uint32_t)(((uint64_t)(w + 1) * T1) / W);
-  uint32_t *C = S.cw + w * T2wp;
-  for (uint32_t e = S.rowstart[r0] + lane; e < S.rowstart[r1]; e += 64) {
+  if (HLDS && PL) {
+    /* the pairs used the screen-table region: put the table back */
+    copy_table<NT>(S.h, J.H32, J.N + 1);
+    __syncthreads();
+  }
+}
+
+/* Adds or removes the pairs of rows [ia, ib) (contiguous in S.nb) into a
+ * wave's column counts. */
+__device__ __forceinline__ void rows_update(const Lds &S, uint32_t *C, uint32_t ia, uint32_t ib, bool add) {
+  const uint32_t lane = lane_id();
+  const uint32_t e1 = S.rowstart[ib];
+  for (uint32_t e = S.rowstart[ia] + lane; e < e1; e += 64) {
     uint32_t b = S.nb[e];
-    atomicAdd(&C[b >> 1], 1u << ((b & 1u) * 16u));
+    uint32_t inc = 1u << ((b & 1u) * 16u);
+    if (add)
+      atomicAdd(&C[b >> 1], inc);
+    else
+      atomicSub(&C[b >> 1], inc);
   }
-  __syncthreads();
-  for (uint32_t t = tid; t < J.T2w; t += NT) {
-    uint32_t run = 0;
-    for (int v = 0; v < W; ++v) {
-      uint32_t x = S.cw[v * T2wp + t];
-      S.cw[v * T2wp + t] = run;
-      run += x;
-    }
-  }
-  __syncthreads();
+  wave_sync();
 }
 
 /* Adds (sign +1) or removes (-1) row i's pairs into a wave's column counts. */
@@ -306,102 +383,358 @@ __device__ __forceinline__ ScreenCell screen_cell(const DevJob &J, const float *
   return c;
 }
 
-/* Upper bound of the log of the reference's computed sum for a sum-class cell. */
-__device__ __forceinline__ float screen_upper(const DevJob &J, uint32_t K, uint32_t n, uint32_t k, float x) {
-  const uint32_t N = J.N;
+/* t_{i+1} / t_i for consecutive terms of the tail (exact mathematics):
+ * (K-i)(n-i) / ((i+1)(N-K-n+i+1)), in FP32.  Relative error <= 7 * 2^-24
+ * (two products, one 1-ulp reciprocal, one product, and the int->float
+ * conversions, exact below 2^24). */
+__device__ __forceinline__ float term_ratio(uint32_t N, uint32_t K, uint32_t n, uint32_t i) {
+  float a = (float)(K - i) * (float)(n - i);
+  float b = (float)(i + 1) * (float)((N - K) - (n - i) + 1);
+  return a * __builtin_amdgcn_rcpf(b);
+}
+
+/* Lower bound of log S, S = sum_{i >= k} t_i / t_k, from one ratio: the
+ * ratios fall with i, so each of the first M+1 terms is at least r^l with
+ * r = ratio at k+M-1, and S >= (1 - r^(M+1)) / (1 - r).  FP32 rounding is
+ * covered by taking 1e-3 off the result. */
+__device__ __forceinline__ float tail_lower_log(uint32_t N, uint32_t K, uint32_t n, uint32_t k) {
   const uint32_t mx = K < n ? K : n;
-  uint64_t num = (uint64_t)(K - k) * (uint64_t)(n - k);
-  uint64_t den = (uint64_t)(k + 1) * (uint64_t)((N - K) - (n - k) + 1);
-  if (num >= den) return 1e-6f; /* p <= 1 (+ rounding) */
-  float lm = __logf((float)(mx - k + 1));
-  float g = fminf(lm, __logf((float)den) - __logf((float)(den - num)));
-  float U = x + J.margin + g + 1e-4f;
-  if (U < -690.f) U = fmaxf(U, lm - 744.44f) + 0.7f; /* subnormal rounding of the terms */
+  const uint32_t M = mx - k < 16u ? mx - k : 16u;
+  if (M == 0u) return 0.f;
+  const float r = term_ratio(N, K, n, k + M - 1u) * (1.f - 16.f * 0x1p-24f);
+  float S;
+  if (r >= 0.9999f)
+    S = 0.99f * (float)(M + 1u); /* each term >= 0.9999^16 > 0.99 */
+  else
+    S = (1.f - exp2f((float)(M + 1u) * __log2f(r))) / (1.f - r);
+  return fmaxf(0.f, __logf(S) - 1e-3f);
+}
+
+/* Bracket of S = sum_{i >= k} t_i / t_k from the ratio recurrence: up to mmax
+ * terms in FP32, then a geometric bound of the rest (the ratio only falls
+ * with i).  Accumulated relative error of the products and the sum is at most
+ * 9 (m+1) 2^-24; hi is +inf when the terms are still growing at the cut. */
+struct TailBracket {
+  float lo, hi;
+};
+__device__ __forceinline__ TailBracket tail_bracket(uint32_t N, uint32_t K, uint32_t n, uint32_t k,
+                                                    uint32_t mmax) {
+  const uint32_t mx = K < n ? K : n;
+  float S = 1.f, P = 1.f;
+  uint32_t i = k, m = 0;
+  while (i < mx && m < mmax) {
+    const float r = term_ratio(N, K, n, i);
+    P *= r;
+    S += P;
+    ++i;
+    ++m;
+    if ((P <= S * 0x1p-30f && r <= 0.9f) || P > 1e30f) break;
+  }
+  const float eps = (float)(9u * (m + 1u)) * 0x1p-24f + 1e-7f;
+  float rem = 0.f;
+  if (i < mx) {
+    const float r = term_ratio(N, K, n, i) * (1.f + 8.f * 0x1p-24f);
+    rem = (r < 0.999f && P <= 1e30f) ? P * (1.f + eps) * r / (1.f - r) * 1.001f : __builtin_inff();
+  }
+  TailBracket t;
+  t.lo = S * (1.f - eps);
+  t.hi = (S + rem) * (1.f + eps);
+  return t;
+}
+
+/* Upper bound of the log of the reference's computed sum for a sum-class
+ * cell, from its FP32 first-term log x. */
+__device__ __forceinline__ float screen_upper(const DevJob &J, uint32_t K, uint32_t n, uint32_t k, float x) {
+  const uint32_t mx = K < n ? K : n;
+  const TailBracket tb = tail_bracket(J.N, K, n, k, 64);
+  if (!(tb.hi < __builtin_inff())) return 1e-6f; /* p <= 1 (+ rounding) */
+  float U = x + J.margin + __logf(tb.hi) + 1e-4f;
+  if (U < -690.f) U = fmaxf(U, __logf((float)(mx - k + 1)) - 744.44f) + 0.7f; /* subnormal term rounding */
   return U;
 }
 
-template <int NT>
-__device__ void flush_candidates(const DevJob &J, const Lds &S, uint2 *cand, uint32_t &nc, BestKey &xb,
-                                 float &Uw) {
+/* The reference's first tail exponent for (K, n, k), FP64 in statrs order. */
+__device__ __forceinline__ double first_exponent(const DevJob &J, uint32_t K, uint32_t n, uint32_t k, double d) {
+  const double *LF = J.LF;
+  const uint32_t N = J.N;
+  double lb1 = (LF[K] - LF[k]) - LF[K - k];
+  double lb2 = (LF[N - K] - LF[n - k]) - LF[(N - K) - (n - k)];
+  return (lb1 + lb2) - d;
+}
+
+struct WaveStats {
+  uint32_t cells, terms, rows;
+  uint64_t cyc[5]; /* build, forward sweep, reverse sweep, tier-1 flushes, exact */
+};
+
+__device__ __forceinline__ uint64_t cycles() { return clock64(); }
+
+/* Exact FP64 evaluation (bit-identical to the reference) of the tier-3 cells
+ * whose lower bound is <= thr. */
+__device__ __forceinline__ void flush_exact(const DevJob &J, const Lds &S, uint4 *c3, uint32_t &n3, float thr, BestKey &xb,
+                            WaveStats &st) {
   const uint32_t lane = lane_id();
   wave_sync();
   BestKey lb = {__builtin_inf(), 0u, kNone};
-  for (uint32_t c = lane; c < nc; c += 64) {
-    uint2 cd = cand[c];
-    uint32_t i = cd.x >> 16, j = cd.x & 0xFFFFu, k = cd.y;
-    double p = dto_tail_exact(J.LF, J.N, J.K[i], S.coln[j], k, J.D[j], S.tab, nullptr);
+  uint32_t cells = 0;
+  for (uint32_t c = lane; c < n3; c += 64) {
+    const uint4 cd = c3[c];
+    if (__uint_as_float(cd.z) > thr) continue;
+    const uint32_t i = cd.x >> 16, j = cd.x & 0xFFFFu, k = cd.y;
+    uint32_t nt = 0;
+    double p = dto_tail_exact(J.LF, J.N, J.K[i], S.coln[j], k, J.D[j], S.tab, &nt);
+    st.terms += nt;
+    ++cells;
     take_better(lb, p, k, cd.x);
   }
+  for (int off = 32; off >= 1; off >>= 1) cells += __shfl_xor(cells, off);
+  st.cells += cells;
   lb = wave_best(lb);
   take_better(xb, lb.p, lb.k, lb.ij);
-  if (xb.ij != kNone) {
-    float lp = xb.p > 0.0 ? (float)(log(xb.p) + 1e-3) : -__builtin_inff();
-    Uw = fminf(Uw, lp);
+  n3 = 0;
+  wave_sync();
+}
+
+__device__ __forceinline__ float log_bound_of(const BestKey &xb) {
+  if (xb.ij == kNone) return __builtin_inff();
+  return xb.p > 0.0 ? (float)(log(xb.p) + 1e-3) : -__builtin_inff();
+}
+
+/* Tier 2 for a full tier-1 list: every lane brackets one cell's log p with
+ * the exact FP64 first exponent and the FP32 ratio recurrence; cells whose
+ * lower end is within the wave's best upper end go to the tier-3 list. */
+template <int NT>
+__device__ __forceinline__ void flush_candidates(const DevJob &J, const Lds &S, uint2 *cand, uint32_t &nc, uint4 *c3,
+                                 uint32_t &n3, BestKey &xb, float &Uw, float &U2w, WaveStats &st) {
+  constexpr int PER = (kCandCap + 63) / 64;
+  const uint32_t lane = lane_id();
+  wave_sync();
+  float L2[PER];
+  float U2min = __builtin_inff();
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const uint32_t c = lane + 64u * r;
+    L2[r] = __builtin_inff();
+    if (c < nc) {
+      const uint2 cd = cand[c];
+      const uint32_t i = cd.x >> 16, j = cd.x & 0xFFFFu, k = cd.y;
+      const uint32_t K = J.K[i], n = S.coln[j];
+      const uint32_t mx = K < n ? K : n;
+      const double x64 = first_exponent(J, K, n, k, J.D[j]);
+      const TailBracket tb = tail_bracket(J.N, K, n, k, 1024);
+      const double lo = x64 + (double)__logf(tb.lo) - 1e-4;
+      L2[r] = lo < -708.0 ? -__builtin_inff() : (float)(lo - 1e-4);
+      if (tb.hi < __builtin_inff()) {
+        double hi = x64 + (double)__logf(tb.hi) + 1e-4;
+        if (hi < -690.0) hi = fmax(hi, log((double)(mx - k + 1)) - 744.44) + 0.7;
+        U2min = fminf(U2min, (float)(hi + 1e-4));
+      } else {
+        U2min = fminf(U2min, 1e-6f);
+      }
+    }
+  }
+  U2w = fminf(U2w, wave_minf(U2min));
+  Uw = fminf(Uw, U2w);
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const uint32_t c = lane + 64u * r;
+    const bool f = c < nc && L2[r] <= U2w;
+    const uint64_t m = __ballot(f);
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    if (cnt) {
+      if (n3 + cnt > (uint32_t)kCand3Cap) {
+        flush_exact(J, S, c3, n3, U2w, xb, st);
+        U2w = fminf(U2w, log_bound_of(xb));
+        Uw = fminf(Uw, U2w);
+      }
+      if (f) {
+        const uint2 cd = cand[c];
+        c3[n3 + mask_lt(m)] = make_uint4(cd.x, cd.y, __float_as_uint(L2[r]), 0u);
+      }
+      n3 += cnt;
+    }
   }
   nc = 0;
   wave_sync();
 }
 
+/* A lane's row segment: kSegWords count words (2 columns each).  With the
+ * reference thresholds T2 <= 640 (N <= ~32k), so a row is one segment and the
+ * lane's column constants live in registers (REGCOLS); wider grids loop over
+ * segments and read the column constants from LDS.  The loads of a segment
+ * are issued together, so a segment costs two LDS round trips. */
+constexpr uint32_t kSegWords = 5;
+constexpr uint32_t kSegCells = 2 * kSegWords;
+
+struct ColRegs {
+  uint32_t n[kSegCells];
+  float b[kSegCells];
+};
+
+/* Reads a segment's count words and returns the lane's count total. */
+__device__ __forceinline__ uint32_t read_segment(const uint32_t *C, uint32_t w0, uint32_t nseg, uint32_t T2w,
+                                                 uint32_t *wv) {
+  uint32_t tot = 0;
+#pragma unroll
+  for (uint32_t u = 0; u < kSegWords; ++u) {
+    wv[u] = 0u;
+    if (u < nseg) {
+      const uint32_t wi = w0 + u;
+      wv[u] = wi < T2w ? C[wi] : 0u;
+      tot += (wv[u] & 0xFFFFu) + (wv[u] >> 16);
+    }
+  }
+  return tot;
+}
+
+/* Overlap k, column size n, FP32 first-term log x and class of each cell of
+ * a segment (class 0 sum, 1 p == 1, 2 p == 0, 3 no cell). */
+template <bool REGCOLS>
+__device__ __forceinline__ void segment_cells(const DevJob &J, const Lds &S, const float *H, const uint32_t *wv,
+                                              uint32_t nseg, uint32_t w0, uint32_t run, uint32_t K, float A,
+                                              const ColRegs &cr, uint32_t *k, uint32_t *nn, float *x,
+                                              uint32_t *cls) {
+  const uint32_t N = J.N, T2 = J.T2;
+  float b[kSegCells], h0[kSegCells], h1[kSegCells], h2[kSegCells], h3[kSegCells];
+#pragma unroll
+  for (uint32_t c = 0; c < kSegCells; ++c) {
+    run += (c & 1u) ? (wv[c >> 1] >> 16) : (wv[c >> 1] & 0xFFFFu);
+    const uint32_t j = 2u * w0 + c;
+    const bool valid = (c >> 1) < nseg && j < T2;
+    k[c] = run;
+    if (REGCOLS) {
+      nn[c] = cr.n[c];
+      b[c] = cr.b[c];
+    } else {
+      const uint32_t jc = valid ? j : 0u;
+      nn[c] = S.coln[jc];
+      b[c] = S.colb[jc];
+    }
+    const uint32_t n = nn[c], kc = k[c];
+    const uint32_t mn = (K + n > N) ? K + n - N : 0u;
+    const uint32_t mx = K < n ? K : n;
+    cls[c] = !valid ? 3u : (kc == 0u || kc <= mn) ? 1u : (kc > mx ? 2u : 0u);
+    const bool sum = cls[c] == 0u;
+    h0[c] = H[sum ? kc : 0u];
+    h1[c] = H[sum ? K - kc : 0u];
+    h2[c] = H[sum ? n - kc : 0u];
+    h3[c] = H[sum ? (N - K) - (n - kc) : 0u];
+  }
+#pragma unroll
+  for (uint32_t c = 0; c < kSegCells; ++c) {
+    x[c] = (A + b[c]) - ((h0[c] + h1[c]) + (h2[c] + h3[c]));
+    if (cls[c] == 0u && x[c] + J.margin < -745.2f) {
+      /* every term from k on is smaller (ratio < 1) and its exponent is below
+       * the exp underflow point -745.1332...: the reference's sum is exactly 0 */
+      const uint32_t kc = k[c], n = nn[c];
+      const uint64_t num = (uint64_t)(K - kc) * (uint64_t)(n - kc);
+      const uint64_t den = (uint64_t)(kc + 1) * (uint64_t)((N - K) - (n - kc) + 1);
+      if (num < den) cls[c] = 2u;
+    }
+  }
+}
+
 /* One permutation: overlap grid -> best cell.  Returns the best key in
- * thread 0 (others: undefined). */
-template <int NT, int RMAX, bool EXPLICIT, bool HLDS, bool PLAIN>
-__device__ BestKey optimize_perm(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
-                                 const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err) {
-  constexpr int W = NT / 64;
+ * thread 0 (others: undefined).  Rows are interleaved over the waves: every
+ * wave adds every row's pairs to its own running column counts, and scans
+ * the rows i with i % W == w. */
+template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS>
+__device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
+                                 const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
+                                 WaveStats &st, uint32_t *gpairs) {
+  constexpr uint32_t W = NT / 64;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
-  build_rows<NT, RMAX, EXPLICIT>(J, Lo, S, i1, i2, s1, s2, err);
+  uint64_t tc0 = cycles();
+  build_rows<NT, EXPLICIT, HLDS, PL>(J, Lo, S, i1, i2, s1, s2, err, gpairs);
+  uint64_t tc1 = cycles();
+  st.cyc[0] += tc1 - tc0;
 
   const uint32_t T1 = J.T1, T2 = J.T2, T2w = J.T2w;
-  const uint32_t r0 = (uint32_t)(((uint64_t)w * T1) / W), r1 = (uint32_t)(((uint64_t)(w + 1) * T1) / W);
   uint32_t *C = S.cw + w * Lo.T2w_pad;
   const uint32_t c2 = (T2w + 63) / 64;
   const uint32_t wbeg = lane * c2;
   const float *H = HLDS ? S.h : J.H32;
+  /* this lane's column constants, kept in registers across rows */
+  ColRegs cr;
+  if (REGCOLS && !PLAIN) {
+#pragma unroll
+    for (uint32_t c = 0; c < kSegCells; ++c) {
+      const uint32_t j = 2u * wbeg + c;
+      const uint32_t jc = (c >> 1) < c2 && j < T2 ? j : 0u;
+      cr.n[c] = S.coln[jc];
+      cr.b[c] = S.colb[jc];
+    }
+  }
 
   BestKey zb = {0.0, 0u, kNone};            /* p == 0 class */
   BestKey ob = {1.0, 0u, kNone};            /* p == 1 class */
   BestKey xb = {__builtin_inf(), 0u, kNone}; /* exactly evaluated cells */
   float Urun = __builtin_inff();
+  /* this lane's cell with the smallest first-term log: its tail bracket,
+   * taken after the sweep, bounds the minimum from above */
+  float bx = __builtin_inff();
+  uint32_t bK = 0, bn = 0, bk = 0;
 
-  /* ---- D: forward sweep ---- */
-  for (uint32_t i = r0; i < r1; ++i) {
-    row_update(S, C, i, true);
-    uint32_t run = row_lane_offset(C, wbeg, c2, T2w);
+  /* ---- D: forward sweep over this wave's rows w, w+W, ... ---- */
+  uint32_t added = 0; /* rows [0, added) are in C */
+  for (uint32_t i = w; i < T1; i += W) {
+    rows_update(S, C, added, i + 1, true);
+    added = i + 1;
     const uint32_t K = J.K[i];
+    if (PLAIN) {
+      uint32_t run = row_lane_offset(C, wbeg, c2, T2w);
+#pragma unroll 1
+      for (uint32_t t = 0; t < c2; ++t) {
+        const uint32_t wi = wbeg + t;
+        const uint32_t x = wi < T2w ? C[wi] : 0u;
+#pragma unroll 1
+        for (uint32_t h = 0; h < 2; ++h) {
+          const uint32_t j = 2 * wi + h;
+          run += h ? (x >> 16) : (x & 0xFFFFu);
+          if (j >= T2) continue;
+          const uint32_t k = run;
+          double p = dto_tail_exact(J.LF, J.N, K, S.coln[j], k, J.D[j], S.tab, nullptr);
+          take_better(xb, p, k, (i << 16) | j);
+        }
+      }
+      continue;
+    }
     const float A = J.A32[i];
     float lmin = __builtin_inff();
-    for (uint32_t t = 0; t < c2; ++t) {
-      const uint32_t wi = wbeg + t;
-      const uint32_t x = wi < T2w ? C[wi] : 0u;
+    uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
+#pragma unroll 1
+    for (uint32_t t0 = 0; t0 < c2; t0 += kSegWords) {
+      const uint32_t nseg = c2 - t0 < kSegWords ? c2 - t0 : kSegWords;
+      uint32_t wv[kSegWords];
+      const uint32_t tot = read_segment(C, wbeg + t0, nseg, T2w, wv);
+      if (REGCOLS) run = wave_excl_scan(tot);
+      uint32_t kk[kSegCells], nn[kSegCells], cls[kSegCells];
+      float xx[kSegCells];
+      segment_cells<REGCOLS>(J, S, H, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn, xx, cls);
+      run += tot;
 #pragma unroll
-      for (uint32_t h = 0; h < 2; ++h) {
-        const uint32_t j = 2 * wi + h;
-        run += h ? (x >> 16) : (x & 0xFFFFu);
-        if (j >= T2) continue;
-        const uint32_t k = run, n = S.coln[j], ij = (i << 16) | j;
-        if (PLAIN) {
-          double p = dto_tail_exact(J.LF, J.N, K, n, k, J.D[j], S.tab, nullptr);
-          take_better(xb, p, k, ij);
-          continue;
-        }
-        ScreenCell c = screen_cell<HLDS>(J, H, K, A, n, S.colb[j], k);
-        if (c.cls == 1) {
-          take_better(ob, 1.0, k, ij);
+      for (uint32_t c = 0; c < kSegCells; ++c) {
+        const uint32_t ij = (i << 16) | (2u * (wbeg + t0) + c);
+        if (cls[c] == 1u) {
+          take_better(ob, 1.0, kk[c], ij);
           Urun = fminf(Urun, 0.f);
-        } else if (c.cls == 2) {
-          take_better(zb, 0.0, k, ij);
+        } else if (cls[c] == 2u) {
+          take_better(zb, 0.0, kk[c], ij);
           Urun = -__builtin_inff();
-        } else {
-          float L = c.x - J.margin;
+        } else if (cls[c] == 0u) {
+          const float L = xx[c] - J.margin;
           lmin = fminf(lmin, L < -708.f ? -__builtin_inff() : L);
-          if (c.x + J.margin < Urun) Urun = fminf(Urun, screen_upper(J, K, n, k, c.x));
+          if (xx[c] < bx) {
+            bx = xx[c];
+            bK = K;
+            bn = nn[c];
+            bk = kk[c];
+          }
         }
       }
     }
-    if (!PLAIN) {
-      lmin = wave_minf(lmin);
-      if (lane == 0) S.rowmin[i] = lmin;
-    }
+    lmin = wave_minf(lmin);
+    if (lane == 0) S.rowmin[i] = lmin;
   }
 
   if (PLAIN) {
@@ -410,69 +743,121 @@ __device__ BestKey optimize_perm(const DevJob &J, const LdsLayout &Lo, const Lds
     __syncthreads();
     BestKey b = {__builtin_inf(), 0u, kNone};
     if (tid == 0)
-      for (int v = 0; v < W; ++v) take_better(b, S.redB[2 * W + v].p, S.redB[2 * W + v].k, S.redB[2 * W + v].ij);
+      for (uint32_t v = 0; v < W; ++v) take_better(b, S.redB[2 * W + v].p, S.redB[2 * W + v].k, S.redB[2 * W + v].ij);
     __syncthreads();
     return b;
   }
 
+  if (bx < __builtin_inff()) Urun = fminf(Urun, screen_upper(J, bK, bn, bk, bx));
   Urun = wave_minf(Urun);
   zb = wave_best(zb);
   ob = wave_best(ob);
+  uint64_t tc2 = cycles();
+  st.cyc[1] += tc2 - tc1;
   if (lane == 0) {
     S.redU[w] = Urun;
     S.redB[w] = zb;
     S.redB[W + w] = ob;
   }
   __syncthreads();
-  float Uw = __builtin_inff();
-  for (int v = 0; v < W; ++v) Uw = fminf(Uw, S.redU[v]);
+  float Ustar = __builtin_inff();
+  for (uint32_t v = 0; v < W; ++v) Ustar = fminf(Ustar, S.redU[v]);
+  float Uw = Ustar;  /* tier-1 threshold (wave, tightens as tier 2 runs) */
+  float U2w = Ustar; /* best tier-2 upper end seen by this wave */
 
-  /* ---- E: reverse sweep over rows that can hold the minimum ---- */
+  /* ---- E: reverse sweep over this wave's rows that can hold the minimum ---- */
   uint2 *cand = S.cand + w * kCandCap;
-  uint32_t nc = 0;
-  for (int32_t ii = (int32_t)r1 - 1; ii >= (int32_t)r0; --ii) {
+  uint4 *c3 = S.cand3 + w * kCand3Cap;
+  uint32_t nc = 0, n3 = 0;
+  /* C holds rows [0, added); the last owned row is added - 1 */
+#pragma unroll 1
+  for (int32_t ii = (int32_t)added - 1; ii >= (int32_t)w; ii -= (int32_t)W) {
     const uint32_t i = (uint32_t)ii;
     if (S.rowmin[i] <= Uw) {
-      uint32_t run = row_lane_offset(C, wbeg, c2, T2w);
+      st.rows++;
       const uint32_t K = J.K[i];
       const float A = J.A32[i];
-      for (uint32_t t = 0; t < c2; ++t) {
-        const uint32_t wi = wbeg + t;
-        const uint32_t x = wi < T2w ? C[wi] : 0u;
+      /* a row whose survivors overflow the list is flushed and rescanned from
+       * the cell that did not fit (cells are visited in a fixed wave-uniform
+       * order, ordinal 2 t0 + c), so every pass makes progress */
+      uint32_t resume = 0;
+#pragma unroll 1
+      for (bool again = true; again;) {
+        again = false;
+        uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
+#pragma unroll 1
+        for (uint32_t t0 = 0; t0 < c2 && !again; t0 += kSegWords) {
+          const uint32_t nseg = c2 - t0 < kSegWords ? c2 - t0 : kSegWords;
+          uint32_t wv[kSegWords];
+          const uint32_t tot = read_segment(C, wbeg + t0, nseg, T2w, wv);
+          if (REGCOLS) run = wave_excl_scan(tot);
+          uint32_t kk[kSegCells], nn[kSegCells], cls[kSegCells];
+          float xx[kSegCells];
+          segment_cells<REGCOLS>(J, S, H, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn, xx, cls);
+          run += tot;
 #pragma unroll
-        for (uint32_t h = 0; h < 2; ++h) {
-          const uint32_t j = 2 * wi + h;
-          run += h ? (x >> 16) : (x & 0xFFFFu);
-          bool f = false;
-          const uint32_t k = run;
-          if (j < T2) {
-            ScreenCell c = screen_cell<HLDS>(J, H, K, A, S.coln[j], S.colb[j], k);
-            if (c.cls == 0) {
-              float L = c.x - J.margin;
+          for (uint32_t c = 0; c < kSegCells; ++c) {
+            const uint32_t ord = 2u * t0 + c;
+            bool f = false;
+            if (cls[c] == 0u && ord >= resume && !again) {
+              float L = xx[c] - J.margin;
+              if (L <= Uw && L >= -708.f) L += tail_lower_log(J.N, K, nn[c], kk[c]);
               L = L < -708.f ? -__builtin_inff() : L;
               f = L <= Uw;
             }
+            const uint64_t m = __ballot(f);
+            const uint32_t cnt = (uint32_t)__popcll(m);
+            if (nc + cnt > (uint32_t)kCandCap) {
+              again = true;
+              resume = ord;
+            } else if (cnt) {
+              if (f) cand[nc + mask_lt(m)] = make_uint2((i << 16) | (2u * (wbeg + t0) + c), kk[c]);
+              nc += cnt;
+            }
           }
-          const uint64_t m = __ballot(f);
-          const uint32_t cnt = (uint32_t)__popcll(m);
-          if (cnt) {
-            if (nc + cnt > (uint32_t)kCandCap) flush_candidates<NT>(J, S, cand, nc, xb, Uw);
-            if (f) cand[nc + mask_lt(m)] = make_uint2((i << 16) | j, k);
-            nc += cnt;
-          }
+        }
+        if (again) {
+          uint64_t tf = cycles();
+          flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
+          st.cyc[3] += cycles() - tf;
         }
       }
     }
-    row_update(S, C, i, false);
+    rows_update(S, C, i >= W ? i - W + 1 : 0u, i + 1, false);
   }
-  if (nc) flush_candidates<NT>(J, S, cand, nc, xb, Uw);
+  uint64_t tc3 = cycles();
+  st.cyc[2] += tc3 - tc2;
+  if (nc) flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
+  uint64_t tc4 = cycles();
+  st.cyc[3] += tc4 - tc3;
+  /* tier 3: exact FP64 for cells whose bracket reaches the best bracket of any wave */
+  if (lane == 0) S.redU2[w] = fminf(U2w, log_bound_of(xb));
+  __syncthreads();
+  uint64_t tc5 = cycles();
+  float U2s = Ustar;
+  for (uint32_t v = 0; v < W; ++v) U2s = fminf(U2s, S.redU2[v]);
+  if (n3) flush_exact(J, S, c3, n3, U2s, xb, st);
+  st.cyc[4] += cycles() - tc5;
   if (lane == 0) S.redB[2 * W + w] = xb;
   __syncthreads();
   BestKey b = {__builtin_inf(), 0u, kNone};
   if (tid == 0)
-    for (int v = 0; v < 3 * W; ++v) take_better(b, S.redB[v].p, S.redB[v].k, S.redB[v].ij);
+    for (uint32_t v = 0; v < 3 * W; ++v) take_better(b, S.redB[v].p, S.redB[v].k, S.redB[v].ij);
   __syncthreads();
   return b;
+}
+
+/* Adds one wave's work counters to the job's counters (one atomic each per wave). */
+__device__ __forceinline__ void flush_stats(const DevJob &J, const WaveStats &st) {
+  uint32_t t = st.terms;
+  for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off);
+  if (lane_id() == 0 && J.stats) {
+    if (st.cells) atomicAdd(&J.stats[0], (unsigned long long)st.cells);
+    if (t) atomicAdd(&J.stats[1], (unsigned long long)t);
+    if (st.rows) atomicAdd(&J.stats[2], (unsigned long long)st.rows);
+    for (int c = 0; c < 5; ++c)
+      if (st.cyc[c]) atomicAdd(&J.stats[3 + c], (unsigned long long)st.cyc[c]);
+  }
 }
 
 __device__ __forceinline__ void write_record(const DevJob &J, const BestKey &b, int32_t permuted,
@@ -503,18 +888,21 @@ __device__ __forceinline__ void write_record(const DevJob &J, const BestKey &b, 
 
 /* Permutations [perm_base, perm_base + n_perm) of one job.  EXPLICIT: idx1 /
  * idx2 hold n_perm permutations back to back (NULL: identity); otherwise the
- * Philox shuffle keyed by (seed, job_id, permutation id, list). */
-template <int NT, int RMAX, bool EXPLICIT, bool HLDS, bool PLAIN>
+ * Philox shuffle keyed by (seed, job_id, permutation id, list).  ws: per
+ * workgroup pair staging (N2 words each) when !PL. */
+template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS>
 __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const uint32_t *__restrict__ idx1,
                                                  const uint32_t *__restrict__ idx2, uint64_t n_perm,
                                                  uint64_t perm_base, uint64_t seed, uint32_t job_id,
                                                  int32_t permuted, dto_record *__restrict__ out,
-                                                 int *__restrict__ err) {
+                                                 int *__restrict__ err, uint32_t *__restrict__ ws) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int W = NT / 64;
   const Lds S = lds_view(smem, Lo, W);
   load_job_tables<NT, HLDS>(J, S);
   __syncthreads();
+  uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * J.N2;
+  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}};
   for (uint64_t pp = blockIdx.x; pp < n_perm; pp += gridDim.x) {
     const uint64_t pid = perm_base + pp;
     const uint32_t *i1 = (EXPLICIT && idx1) ? idx1 + pp * J.N1 : nullptr;
@@ -524,9 +912,10 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
       s1 = shuffle_make(seed, job_id, (uint32_t)pid, 1u, J.N1);
       s2 = shuffle_make(seed, job_id, (uint32_t)pid, 2u, J.N2);
     }
-    BestKey b = optimize_perm<NT, RMAX, EXPLICIT, HLDS, PLAIN>(J, Lo, S, i1, i2, s1, s2, err);
+    BestKey b = optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS>(J, Lo, S, i1, i2, s1, s2, err, st, gpairs);
     if (threadIdx.x == 0) write_record(J, b, permuted, out + pp, err);
   }
+  flush_stats(J, st);
 }
 
 /* ------------------------------------------------------------------ */
@@ -561,24 +950,33 @@ static __global__ void k_reset_counter(uint32_t *counter, int *err) {
  * (one atomic counter) until it is drained; a workgroup reloads the per-job
  * LDS tables only when it moves to another job.  Every workgroup exits once
  * the counter passes n_items. */
-template <int NT, int RMAX, bool HLDS>
+template <int NT, bool HLDS, bool PL, bool REGCOLS>
 __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs, const WorkItem *__restrict__ items,
                                               uint32_t n_items, uint64_t seed, uint32_t *__restrict__ counter,
-                                              int *__restrict__ err) {
+                                              int *__restrict__ err, uint32_t *__restrict__ ws, uint32_t ws_stride) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int W = NT / 64;
   __shared__ uint32_t s_item;
+  uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * ws_stride;
   uint32_t cur = kNone;
+  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}};
+  const DevJob *last = nullptr;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
     __syncthreads();
     const uint32_t it = s_item;
     __syncthreads();
-    if (it >= n_items) break;
+    if (it >= n_items) {
+      if (last) flush_stats(*last, st);
+      break;
+    }
     const WorkItem wi = items[it];
     const BatchJob &B = jobs[wi.job];
     const Lds S = lds_view(smem, B.lo, W);
     if (wi.job != cur) {
+      if (last) flush_stats(*last, st);
+      st = {0u, 0u, 0u, {0, 0, 0, 0, 0}};
+      last = &B.J;
       load_job_tables<NT, HLDS>(B.J, S);
       __syncthreads();
       cur = wi.job;
@@ -587,11 +985,11 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs,
       BestKey b;
       Shuffle s1, s2;
       if (t == 0) {
-        b = optimize_perm<NT, RMAX, true, HLDS, false>(B.J, B.lo, S, nullptr, nullptr, s1, s2, err);
+        b = optimize_perm<NT, true, HLDS, false, PL, REGCOLS>(B.J, B.lo, S, nullptr, nullptr, s1, s2, err, st, gpairs);
       } else {
         s1 = shuffle_make(seed, B.job_id, t - 1, 1u, B.J.N1);
         s2 = shuffle_make(seed, B.job_id, t - 1, 2u, B.J.N2);
-        b = optimize_perm<NT, RMAX, false, HLDS, false>(B.J, B.lo, S, nullptr, nullptr, s1, s2, err);
+        b = optimize_perm<NT, false, HLDS, false, PL, REGCOLS>(B.J, B.lo, S, nullptr, nullptr, s1, s2, err, st, gpairs);
       }
       if (threadIdx.x == 0) write_record(B.J, b, t == 0 ? 0 : 1, B.out + t, err);
     }
@@ -599,22 +997,23 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs,
 }
 
 /* Overlap counts of every cell for one permutation (debug grid export). */
-template <int NT, int RMAX>
+template <int NT, bool PL>
 __global__ __launch_bounds__(NT) void k_grid_counts(DevJob J, LdsLayout Lo, const uint32_t *__restrict__ idx1,
                                                     const uint32_t *__restrict__ idx2,
-                                                    uint32_t *__restrict__ kout, int *__restrict__ err) {
+                                                    uint32_t *__restrict__ kout, int *__restrict__ err,
+                                                    uint32_t *__restrict__ ws) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int W = NT / 64;
   const Lds S = lds_view(smem, Lo, W);
   Shuffle s1, s2;
-  build_rows<NT, RMAX, true>(J, Lo, S, idx1, idx2, s1, s2, err);
+  build_rows<NT, true, false, PL>(J, Lo, S, idx1, idx2, s1, s2, err, ws);
   const uint32_t w = threadIdx.x >> 6, lane = lane_id();
   const uint32_t T1 = J.T1, T2 = J.T2, T2w = J.T2w;
-  const uint32_t r0 = (uint32_t)(((uint64_t)w * T1) / W), r1 = (uint32_t)(((uint64_t)(w + 1) * T1) / W);
   uint32_t *C = S.cw + w * Lo.T2w_pad;
   const uint32_t c2 = (T2w + 63) / 64, wbeg = lane * c2;
-  for (uint32_t i = r0; i < r1; ++i) {
+  for (uint32_t i = 0; i < T1; ++i) {
     row_update(S, C, i, true);
+    if (i % W != w) continue;
     uint32_t run = row_lane_offset(C, wbeg, c2, T2w);
     for (uint32_t t = 0; t < c2; ++t) {
       const uint32_t wi = wbeg + t;

@@ -142,13 +142,14 @@ DTO_HD double dto_exp(double x, const uint64_t *tab) {
  * precomputed in the same order.
  *
  * Early exit: after adding term i, stop when term_i * 2^56 <= acc and the
- * exact term ratio (K-i)(n-i) / ((i+1)(N-K-n+i+1)) is <= 1/2.  The ratio
- * decreases with i, so every later term is below term_i / 2^(j-i) up to the
- * ~1e-9 relative error of the computed exponents; each is therefore below
- * half an ulp of acc and leaves acc unchanged under round-to-nearest, exactly
- * as if it had been added.  A zero term (exp underflow) with the ratio
- * condition stops too: every later exponent is smaller and also underflows.
- * Returns the number of terms evaluated through *terms (may be NULL). */
+ * exact term ratio rho_i = (K-i)(n-i) / ((i+1)(N-K-n+i+1)) is <= 1 - 2^-20.
+ * rho decreases with i, so every later exact exponent is below exponent i by
+ * at least 2^-20 nats, far more than the ~1e-8 error of the computed
+ * exponents; the computed later terms are then no larger than term_i (up to
+ * one ulp of exp's own rounding, or one subnormal step), hence below half an
+ * ulp of acc, and leave acc unchanged under round-to-nearest exactly as if
+ * they had been added.  When term_i underflowed to 0 every later term
+ * underflows too.  Returns the number of terms evaluated through *terms. */
 DTO_HD double dto_tail_exact(const double *LF, uint32_t N, uint32_t K, uint32_t n, uint32_t k,
                              double d, const uint64_t *tab, uint32_t *terms) {
   if (terms) *terms = 0;
@@ -169,9 +170,10 @@ DTO_HD double dto_tail_exact(const double *LF, uint32_t N, uint32_t K, uint32_t 
     double t = dto_exp((lb1 + lb2) - d, tab);
     acc = acc + t;
     if (t * 0x1p56 <= acc) {
-      uint64_t num = 2ull * (uint64_t)(K - i) * (uint64_t)(n - i);
+      /* rho_i <= 1 - 2^-20, checked as num <= den - ceil(den / 2^20) */
+      uint64_t num = (uint64_t)(K - i) * (uint64_t)(n - i);
       uint64_t den = (uint64_t)(i + 1) * (uint64_t)(NK - n + i + 1);
-      if (num <= den) {
+      if (num <= den - ((den + 0xFFFFFull) >> 20)) {
         ++i;
         break;
       }
