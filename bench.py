@@ -146,7 +146,9 @@ def main():
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": read_traffic(args.traffic_file),
-                "kernel": "k_optimize<512,64,explicit,lds-table,screened>", "kernel_ms": round(kern_ms, 4),
+                "kernel": "k_optimize<512 threads, explicit permutations, H table in LDS, screened, pairs in LDS, "
+                          "column constants in registers>",
+                "kernel_ms": round(kern_ms, 4),
                 "algorithmic_bytes_per_launch": bytes_per_launch}
 
     result = {"metric": METRIC, "value": round(value, 2), "unit": "permutations/s", "n_gpus": 1,

@@ -139,6 +139,7 @@ INTERNAL = {
     "dto_internal_exp_host": (_int, [_P(_dbl), _u64, _P(_dbl)]),
     "dto_internal_shuffle_device": (_int, [_u64, _u32, _u32, _u32, _u32, _P(_u32)]),
     "dto_internal_stats": (_int, [_vp, _P(_u64), _int]),
+    "dto_internal_philox": (_int, [_P(_u32), _P(_u32), _P(_u32)]),
 }
 
 _lib = None

@@ -385,6 +385,22 @@ int dto_internal_exp_host(const double *x, uint64_t count, double *out) {
   return DTO_OK;
 }
 
+/* one Philox4x32-10 block (host), for the published known-answer vectors */
+int dto_internal_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+  if (!ctr || !key || !out) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  U32x4 c;
+  c.x = ctr[0];
+  c.y = ctr[1];
+  c.z = ctr[2];
+  c.w = ctr[3];
+  const U32x4 r = philox4x32_10(c, key[0], key[1]);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+  out[3] = r.w;
+  return DTO_OK;
+}
+
 int dto_internal_shuffle_device(uint64_t seed, uint32_t job, uint32_t perm, uint32_t list, uint32_t n, uint32_t *out) {
   int dev, rc;
   if ((rc = current_device(&dev))) return rc;
