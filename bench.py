@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--perms", type=int, default=10000)
     p.add_argument("--plain-perms", type=int, default=256)
+    p.add_argument("--dense-perms", type=int, default=2000)
+    p.add_argument("--dense-plain-perms", type=int, default=64)
     p.add_argument("--batch-jobs", type=int, default=512)
     p.add_argument("--batch-perms", type=int, default=1000)
     p.add_argument("--serving-requests", type=int, default=400)
@@ -190,6 +192,41 @@ def main():
                             "screened_equals_plain_on_%d_perms" % Pp: same}
     del idx1, idx2
     torch.cuda.empty_cache()
+
+    # ---- dense thresholds: 2,000 x 2,000 custom thresholds on the same job (4M cells) ----
+    if args.dense_perms > 0:
+        t1, t2 = W.dense_2k_thresholds()
+        dj = dto.Job.from_lists(ids1, r1, ids2, r2, background=bg, drop_outside_background=True)
+        dj.set_thresholds(t1, t2)
+        DP = args.dense_perms
+        dout = torch.empty(DP * RECORD_BYTES, dtype=torch.uint8, device=dev)
+        dj.run_device(DP, dout.data_ptr(), seed=4, stream=sptr)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        dj.run_device(DP, dout.data_ptr(), seed=4, stream=sptr)
+        torch.cuda.synchronize()
+        dense_rate = DP / (time.perf_counter() - t)
+        rs = np.frombuffer(dout.cpu().numpy().tobytes(), dtype=dto._lib.RECORD_DTYPE).copy()
+        DS = min(args.dense_plain_perms, DP)
+        t = time.perf_counter()
+        dj.run_device(DS, dout.data_ptr(), seed=4, mode="plain", stream=sptr)
+        torch.cuda.synchronize()
+        dense_plain_rate = DS / (time.perf_counter() - t)
+        rp = np.frombuffer(dout[:DS * RECORD_BYTES].cpu().numpy().tobytes(), dtype=dto._lib.RECORD_DTYPE)
+        ps, pp = rs["pvalue"][:DS], rp["pvalue"]
+        rel = np.where(pp > 0, np.abs(ps - pp) / np.where(pp > 0, pp, 1.0), np.abs(ps - pp))
+        agree = float(np.mean((rs["idx1"][:DS] == rp["idx1"]) & (rs["idx2"][:DS] == rp["idx2"])))
+        log("dense 2k x 2k: screened %.0f perms/s, plain FP64 %.1f perms/s; max rel err of min p %.3g, "
+            "argmin agreement %.4f on %d permutations" % (dense_rate, dense_plain_rate, float(rel.max()), agree, DS))
+        result["dense_2k_thresholds"] = {
+            "workload": "genome_20k job with thresholds 10, 20, ..., 20,000 on both lists (4,000,000 cells), "
+                        "Philox permutations",
+            "screened_perms_per_s": round(dense_rate, 2), "plain_fp64_perms_per_s": round(dense_plain_rate, 3),
+            "screened_vs_plain": round(dense_rate / dense_plain_rate, 2),
+            "max_rel_err_min_p_vs_fp64": float(rel.max()), "argmin_agreement_vs_fp64": agree,
+            "compared_permutations": DS}
+        del dout
+        dj.close()
 
     # ---- batched: 512 uneven jobs in one persistent launch vs one launch per job ----
     if args.batch_jobs > 0:

@@ -535,23 +535,30 @@ __device__ __forceinline__ void flush_candidates(const DevJob &J, const Lds &S, 
   }
   U2w = fminf(U2w, wave_minf(U2min));
   Uw = fminf(Uw, U2w);
+  /* survivors move to the tier-3 list half a wave at a time: at most 32
+   * entries per push, so a push after a flush always fits the list */
+  static_assert(kCand3Cap >= 32, "a half-wave push must fit an empty tier-3 list");
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
     const uint32_t c = lane + 64u * r;
     const bool f = c < nc && L2[r] <= U2w;
-    const uint64_t m = __ballot(f);
-    const uint32_t cnt = (uint32_t)__popcll(m);
-    if (cnt) {
-      if (n3 + cnt > (uint32_t)kCand3Cap) {
-        flush_exact(J, S, c3, n3, U2w, xb, st);
-        U2w = fminf(U2w, log_bound_of(xb));
-        Uw = fminf(Uw, U2w);
+#pragma unroll
+    for (uint32_t half = 0; half < 2; ++half) {
+      const bool fh = f && (lane >> 5) == half;
+      const uint64_t m = __ballot(fh);
+      const uint32_t cnt = (uint32_t)__popcll(m);
+      if (cnt) {
+        if (n3 + cnt > (uint32_t)kCand3Cap) {
+          flush_exact(J, S, c3, n3, U2w, xb, st);
+          U2w = fminf(U2w, log_bound_of(xb));
+          Uw = fminf(Uw, U2w);
+        }
+        if (fh) {
+          const uint2 cd = cand[c];
+          c3[n3 + mask_lt(m)] = make_uint4(cd.x, cd.y, __float_as_uint(L2[r]), 0u);
+        }
+        n3 += cnt;
       }
-      if (f) {
-        const uint2 cd = cand[c];
-        c3[n3 + mask_lt(m)] = make_uint4(cd.x, cd.y, __float_as_uint(L2[r]), 0u);
-      }
-      n3 += cnt;
     }
   }
   nc = 0;
@@ -766,6 +773,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   float U2w = Ustar; /* best tier-2 upper end seen by this wave */
 
   /* ---- E: reverse sweep over this wave's rows that can hold the minimum ---- */
+  static_assert(kCandCap >= 64, "one ballot (64 cells) must fit an empty tier-1 list");
   uint2 *cand = S.cand + w * kCandCap;
   uint4 *c3 = S.cand3 + w * kCand3Cap;
   uint32_t nc = 0, n3 = 0;

@@ -176,6 +176,25 @@ or_list *or_list_new(const char *const *ids, const uint32_t *ranks, uint64_t n) 
   return l;
 }
 
+/* Custom threshold values (increasing), replacing generate_thresholds: the
+ * dense-grid configuration has no reference API (thresholds are private,
+ * ranked.rs:132), so the oracle takes them as given and keeps the reference's
+ * per-threshold set definition, ranks <= t (ranked.rs:561-568). */
+int or_list_set_thresholds(or_list *l, const uint32_t *thr, uint32_t nt) {
+  for (uint32_t t = 1; t < nt; t++)
+    if (thr[t] <= thr[t - 1]) return -1;
+  free(l->thr);
+  l->nt = nt;
+  l->thr = calloc(nt ? nt : 1, sizeof(uint32_t));
+  memcpy(l->thr, thr, (size_t)nt * sizeof(uint32_t));
+  uint32_t t = 0;
+  for (uint64_t p = 0; p < l->n; p++) {
+    while (t < l->nt && l->thr[t] < l->ranks[p]) t++;
+    l->bin[p] = t;
+  }
+  return 0;
+}
+
 void or_list_free(or_list *l) {
   if (!l) return;
   for (uint64_t i = 0; i < l->n; i++) free(l->ids[i]);

@@ -59,6 +59,8 @@ typedef struct or_list or_list;
 /* RankedFeatureList::from(genes, ranks): copies ids, stable-sorts by rank and
  * builds thresholds. */
 or_list *or_list_new(const char *const *ids, const uint32_t *ranks, uint64_t n);
+/* Replace the list's thresholds with increasing custom values (0 ok, -1 not increasing). */
+int or_list_set_thresholds(or_list *l, const uint32_t *thr, uint32_t nt);
 void or_list_free(or_list *l);
 uint64_t or_list_len(const or_list *l);
 uint32_t or_list_n_thresholds(const or_list *l);

@@ -57,6 +57,8 @@ def lib():
         L.or_generate_thresholds.argtypes = [u32, P(u32), u32]; L.or_generate_thresholds.restype = u32
         L.or_list_new.argtypes = [P(ctypes.c_char_p), P(u32), u64]; L.or_list_new.restype = ctypes.c_void_p
         L.or_list_free.argtypes = [ctypes.c_void_p]
+        L.or_list_set_thresholds.argtypes = [ctypes.c_void_p, P(u32), u32]
+        L.or_list_set_thresholds.restype = ctypes.c_int
         L.or_list_len.argtypes = [ctypes.c_void_p]; L.or_list_len.restype = u64
         L.or_list_n_thresholds.argtypes = [ctypes.c_void_p]; L.or_list_n_thresholds.restype = u32
         L.or_list_thresholds.argtypes = [ctypes.c_void_p]; L.or_list_thresholds.restype = P(u32)
@@ -157,6 +159,11 @@ class OracleList:
         if getattr(self, "h", None) and _lib is not None:
             _lib.or_list_free(self.h)
             self.h = None
+
+    def set_thresholds(self, thr):
+        t = np.ascontiguousarray(thr, dtype=np.uint32)
+        if lib().or_list_set_thresholds(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(t)):
+            raise ValueError("thresholds must increase")
 
     @property
     def thresholds(self):

@@ -194,6 +194,66 @@ def test_screened_equals_plain_genome_20k(dto, gpu):
     assert rs.empirical_pvalue == rp.empirical_pvalue
 
 
+def test_dense_thresholds_vs_oracle(dto, O, gpu):
+    """Custom threshold grids wider than one register segment per lane
+    (column constants read from LDS), against the oracle with the same
+    thresholds, in both modes."""
+    from dual_threshold_optimization_amd import workloads as W
+    for n, step, planted in [(2500, 2, 120), (1400, 1, 0), (1800, 3, 60)]:
+        ids1, r1, ids2, r2, _ = W.planted_lists(n, planted, 300, 250, 0.05, seed=n)
+        job = dto.Job.from_lists(ids1, r1, ids2, r2)
+        t1 = np.arange(step, n + 1, step, dtype=np.uint32)
+        t2 = np.arange(1, n + 1, max(1, step - 1), dtype=np.uint32)
+        job.set_thresholds(t1, t2)
+        assert (job.n_thresholds1, job.n_thresholds2) == (len(t1), len(t2))
+        l1, l2 = _oracle_pair(O, ids1, r1, ids2, r2)
+        l1.set_thresholds(t1)
+        l2.set_thresholds(t2)
+        P = 3
+        i1, i2 = W.explicit_permutations(P, job.len1, job.len2, 13)
+        ub = O.optimize(l1, l2, n, fast=True)
+        ob = O.optimize_many(l1, l2, n, i1, i2, fast=True)
+        for mode in ("screened", "plain"):
+            res, perm = job.run(P, idx1=i1, idx2=i2, mode=mode)
+            u = res.unpermuted
+            assert (u.rank1, u.rank2, u.set1_len, u.set2_len, u.population_size, u.intersection_size,
+                    u.pvalue) == ub.as_tuple(), (n, mode)
+            for p in range(P):
+                assert rec_tuple(perm[p]) == ob[p].as_tuple(), (n, mode, p)
+
+
+def test_dense_2k_genome_20k_fixture(dto, gpu):
+    """4M-cell grids (thresholds 10..20,000 step 10 on the genome-scale job):
+    the committed oracle records (tests/golden/make_dense_fixture.py), then
+    screened == plain on further permutations."""
+    import json
+    import os
+    from dual_threshold_optimization_amd import workloads as W
+    from helpers import GOLDEN
+    with open(os.path.join(GOLDEN, "dense_2k_genome_20k.json")) as f:
+        fx = json.load(f)
+    ids1, r1, ids2, r2, bg = W.genome_20k()
+    job = dto.Job.from_lists(ids1, r1, ids2, r2, background=bg, drop_outside_background=True)
+    t1, t2 = W.dense_2k_thresholds()
+    job.set_thresholds(t1, t2)
+    P = len(fx["permuted"])
+    i1, i2 = W.explicit_permutations(P, job.len1, job.len2, 21)
+
+    def dec(d):
+        return (d["rank1"], d["rank2"], d["set1_len"], d["set2_len"], d["population_size"],
+                d["intersection_size"], float.fromhex(d["pvalue"]))
+    for mode in ("screened", "plain"):
+        res, perm = job.run(P, idx1=i1, idx2=i2, mode=mode)
+        u = res.unpermuted
+        assert (u.rank1, u.rank2, u.set1_len, u.set2_len, u.population_size, u.intersection_size,
+                u.pvalue) == dec(fx["unpermuted"]), mode
+        for p in range(P):
+            assert rec_tuple(perm[p]) == dec(fx["permuted"][p]), (mode, p)
+    rs, ps = job.run(24, seed=31, mode="screened")
+    rp, pp = job.run(24, seed=31, mode="plain")
+    assert [rec_tuple(x) for x in ps] == [rec_tuple(x) for x in pp]
+
+
 def test_batch_vs_oracle_and_graph_replay(dto, O, gpu):
     from dual_threshold_optimization_amd import workloads as W
     specs = W.batch_jobs(n_jobs=10, seed=9, n_min=60, n_max=1500)
