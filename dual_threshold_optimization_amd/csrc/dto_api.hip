@@ -65,46 +65,43 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
   const uint32_t T1 = (uint32_t)h.thr1.size(), T2 = (uint32_t)h.thr2.size();
   const uint32_t T2w = (T2 + 1) / 2;
   L.T2w_pad = (T2w + 3u) & ~3u;
-  uint32_t o = 0;
-  /* fixed regions first */
-  L.nb = o;
-  o = align16(o + 2u * std::max(std::max(N1, N2), 1u));
-  L.rowstart = o;
-  o = align16(o + 4u * (T1 + 1));
-  L.cursor = o;
-  o = align16(o + 4u * T1);
-  L.cw = o;
-  o = align16(o + 4u * (uint32_t)W * L.T2w_pad);
-  L.rowmin = o;
-  o = align16(o + 4u * T1);
-  L.coln = o;
-  o = align16(o + 4u * T2);
-  L.colb = o;
-  o = align16(o + 4u * T2);
-  L.cand = o;
-  o = align16(o + 8u * (uint32_t)W * (uint32_t)kCandCap);
-  L.cand3 = o;
-  o = align16(o + 16u * (uint32_t)W * (uint32_t)kCand3Cap);
-  L.exptab = o;
-  o = align16(o + 8u * 256u);
-  L.red = o;
-  o = align16(o + 80u * (uint32_t)W);
-  /* screen table and pair staging: the pairs reuse the table's region */
-  L.h = L.pairs = o;
+  /* the screen table (or, without it, the pair staging) sits at offset 0 so
+   * that table reads address it with the scaled index alone */
   const uint32_t hb = with_h ? 4u * (uint32_t)(h.N + 1) : 0u, pb = 4u * std::max(N2, 1u);
+  uint32_t fixed = 0;
+  auto region = [&](uint32_t &field, uint32_t bytes) {
+    field = fixed;
+    fixed = align16(fixed + bytes);
+  };
+  region(L.nb, 2u * std::max(std::max(N1, N2), 1u));
+  region(L.rowstart, 4u * (T1 + 1));
+  region(L.cursor, 4u * T1);
+  region(L.cw, 4u * (uint32_t)W * L.T2w_pad);
+  region(L.rowmin, 4u * T1);
+  region(L.coln, 4u * T2);
+  region(L.colb, 4u * T2);
+  region(L.cand, 8u * (uint32_t)W * (uint32_t)kCandCap);
+  region(L.cand3, 16u * (uint32_t)W * (uint32_t)kCand3Cap);
+  region(L.exptab, 8u * 256u);
+  region(L.red, 80u * (uint32_t)W);
+  uint32_t base = 0;
+  L.h = L.pairs = 0;
   if (with_h) {
     L.h_in_lds = 1;
     L.pairs_in_lds = 1;
-    o = align16(o + std::max(hb, pb));
-  } else if (o + pb <= lds_limit) {
+    base = align16(std::max(hb, pb));
+  } else if (align16(pb) + fixed <= lds_limit) {
     L.h_in_lds = 0;
     L.pairs_in_lds = 1;
-    o = align16(o + pb);
+    base = align16(pb);
   } else {
     L.h_in_lds = 0;
     L.pairs_in_lds = 0;
   }
-  L.total = o;
+  for (uint32_t *f : {&L.nb, &L.rowstart, &L.cursor, &L.cw, &L.rowmin, &L.coln, &L.colb, &L.cand, &L.cand3,
+                      &L.exptab, &L.red})
+    *f += base;
+  L.total = base + fixed;
   return L;
 }
 
@@ -129,7 +126,7 @@ int upload_job(dto_job *j) {
   size_t o_t1 = put(buf, h.thr1), o_t2 = put(buf, h.thr2), o_LF = put(buf, h.LF), o_D = put(buf, h.D);
   size_t o_H = put(buf, h.H32), o_A = put(buf, h.A32), o_B = put(buf, h.B32);
   size_t o_err = put(buf, std::vector<int>(4, 0));
-  size_t o_stats = put(buf, std::vector<unsigned long long>(8, 0));
+  size_t o_stats = put(buf, std::vector<unsigned long long>(12, 0));
   if (j->arena) {
     (void)hipFree(j->arena);
     j->arena = nullptr;
@@ -413,13 +410,15 @@ int dto_internal_shuffle_device(uint64_t seed, uint32_t job, uint32_t perm, uint
   return DTO_OK;
 }
 
-/* work counters of a job's screened launches: exact cells, exact terms,
- * rescanned rows (test / profiling hook, not in dto.h) */
+/* work counters of a job's screened launches (test / profiling hook, not in
+ * dto.h), 12 values: exact cells, exact terms, rescanned rows, cycles of the
+ * build / forward / reverse / tier-1 flush / exact phases, tier-1 pushes,
+ * tier-2 survivors, tier-1 flushes */
 int dto_internal_stats(dto_job *job, uint64_t *out, int reset) {
   if (!job || !out) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
   HIP_TRY(hipDeviceSynchronize());
-  HIP_TRY(hipMemcpy(out, job->dev.stats, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost));
-  if (reset) HIP_TRY(hipMemset(job->dev.stats, 0, 8 * sizeof(uint64_t)));
+  HIP_TRY(hipMemcpy(out, job->dev.stats, 12 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if (reset) HIP_TRY(hipMemset(job->dev.stats, 0, 12 * sizeof(uint64_t)));
   return DTO_OK;
 }
 

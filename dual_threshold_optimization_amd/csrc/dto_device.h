@@ -35,7 +35,7 @@ struct DevJob {
   const float *H32;         /* [N+1] LF[x] - x*(ln N - 1), rounded to f32 */
   const float *A32;         /* [T1] H(K_i) + H(N-K_i) */
   const float *B32;         /* [T2] H(n_j) + H(N-n_j) - H(N) */
-  unsigned long long *stats; /* [8] work counters: 0 exact cells, 1 exact terms, 2 rows rescanned */
+  unsigned long long *stats; /* [12] work counters (dto_internal_stats) */
 };
 
 /* Byte offsets of the dynamic LDS regions of one workgroup. */

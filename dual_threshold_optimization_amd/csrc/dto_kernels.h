@@ -305,17 +305,24 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
 }
 
 /* Adds or removes the pairs of rows [ia, ib) (contiguous in S.nb) into a
- * wave's column counts. */
+ * wave's column counts (four bins in flight per lane). */
 __device__ __forceinline__ void rows_update(const Lds &S, uint32_t *C, uint32_t ia, uint32_t ib, bool add) {
   const uint32_t lane = lane_id();
   const uint32_t e1 = S.rowstart[ib];
-  for (uint32_t e = S.rowstart[ia] + lane; e < e1; e += 64) {
-    uint32_t b = S.nb[e];
-    uint32_t inc = 1u << ((b & 1u) * 16u);
-    if (add)
-      atomicAdd(&C[b >> 1], inc);
-    else
-      atomicSub(&C[b >> 1], inc);
+  for (uint32_t e = S.rowstart[ia] + lane; e < e1; e += 256) {
+    uint32_t b[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) b[u] = e + 64u * u < e1 ? (uint32_t)S.nb[e + 64u * u] : 0xFFFFu;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      if (b[u] != 0xFFFFu) {
+        const uint32_t inc = 1u << ((b[u] & 1u) * 16u);
+        if (add)
+          atomicAdd(&C[b[u] >> 1], inc);
+        else
+          atomicSub(&C[b[u] >> 1], inc);
+      }
+    }
   }
   wave_sync();
 }
@@ -346,41 +353,6 @@ __device__ __forceinline__ uint32_t row_lane_offset(const uint32_t *C, uint32_t 
     }
   }
   return wave_excl_scan(tot);
-}
-
-struct ScreenCell {
-  int cls; /* 0 sum, 1 p == 1 exactly, 2 p == 0 exactly */
-  float x; /* FP32 estimate of the first term's log (sum class) */
-};
-
-/* Classify one cell and form the FP32 log of its first tail term. */
-template <bool HLDS>
-__device__ __forceinline__ ScreenCell screen_cell(const DevJob &J, const float *H, uint32_t K, float A,
-                                                  uint32_t n, float B, uint32_t k) {
-  ScreenCell c;
-  const uint32_t N = J.N;
-  const uint32_t mn = (K + n > N) ? K + n - N : 0u;
-  const uint32_t mx = K < n ? K : n;
-  c.x = 0.f;
-  if (k == 0 || k <= mn) {
-    c.cls = 1; /* k == 0 -> 1.0; k - 1 < min -> sf = 1.0 */
-    return c;
-  }
-  if (k > mx) {
-    c.cls = 2; /* k - 1 >= max -> sf = 0.0 (duplicate list2 ids only) */
-    return c;
-  }
-  float s = (H[k] + H[K - k]) + (H[n - k] + H[(N - K) - (n - k)]);
-  c.x = (A + B) - s;
-  c.cls = 0;
-  if (c.x + J.margin < -745.2f) {
-    /* every term from k on is smaller (ratio < 1) and its exponent is below
-     * the exp underflow point -745.1332...: the reference's sum is exactly 0 */
-    uint64_t num = (uint64_t)(K - k) * (uint64_t)(n - k);
-    uint64_t den = (uint64_t)(k + 1) * (uint64_t)((N - K) - (n - k) + 1);
-    if (num < den) c.cls = 2;
-  }
-  return c;
 }
 
 /* t_{i+1} / t_i for consecutive terms of the tail (exact mathematics):
@@ -465,6 +437,8 @@ __device__ __forceinline__ double first_exponent(const DevJob &J, uint32_t K, ui
 struct WaveStats {
   uint32_t cells, terms, rows;
   uint64_t cyc[5]; /* build, forward sweep, reverse sweep, tier-1 flushes, exact */
+  uint32_t push1, push2, flushes; /* tier-1 pushes, tier-2 survivors, tier-1 list flushes */
+  uint32_t gap;                    /* sum of (first upper bound - log of the minimum), milli-nats */
 };
 
 __device__ __forceinline__ uint64_t cycles() { return clock64(); }
@@ -558,10 +532,12 @@ __device__ __forceinline__ void flush_candidates(const DevJob &J, const Lds &S, 
           c3[n3 + mask_lt(m)] = make_uint4(cd.x, cd.y, __float_as_uint(L2[r]), 0u);
         }
         n3 += cnt;
+        st.push2 += cnt;
       }
     }
   }
   nc = 0;
+  st.flushes++;
   wave_sync();
 }
 
@@ -573,72 +549,167 @@ __device__ __forceinline__ void flush_candidates(const DevJob &J, const Lds &S, 
 constexpr uint32_t kSegWords = 5;
 constexpr uint32_t kSegCells = 2 * kSegWords;
 
+/* Column constant of a cell outside the sum class or past the last column:
+ * its screen value becomes ~3e38, above every threshold, and stays finite so
+ * that the packed minimum below never meets a NaN. */
+constexpr float kNoCell = 3.0e38f;
+
+/* The low kPackBits mantissa bits of a lane's running minimum carry the
+ * index of the cell that holds it; this moves the value by at most
+ * 2^kPackBits ulp, i.e. |x| 2^(kPackBits - 23), which every bound derived
+ * from a packed value adds back (pack_slack). */
+constexpr uint32_t kPackBits = 4;
+static_assert(kSegCells <= (1u << kPackBits), "a segment's cell index must fit the packed bits");
+__device__ __forceinline__ float pack_cell(float x, uint32_t idx) {
+  return __int_as_float((__float_as_int(x) & ~((1 << kPackBits) - 1)) | (int)idx);
+}
+__device__ __forceinline__ uint32_t packed_cell(float v) {
+  return (uint32_t)__float_as_int(v) & ((1u << kPackBits) - 1u);
+}
+__device__ __forceinline__ float pack_slack(float v) { return fabsf(v) * 0x1p-18f + 1e-30f; }
+
 struct ColRegs {
-  uint32_t n[kSegCells];
+  uint32_t n4[kSegCells]; /* 4 n_j */
   float b[kSegCells];
 };
 
-/* Reads a segment's count words and returns the lane's count total. */
+/* Reads a segment's count words (all loads issued together, indices
+ * clamped into the wave's row of T2w words) and returns the lane's count
+ * total. */
 __device__ __forceinline__ uint32_t read_segment(const uint32_t *C, uint32_t w0, uint32_t nseg, uint32_t T2w,
                                                  uint32_t *wv) {
+  uint32_t raw[kSegWords];
+#pragma unroll
+  for (uint32_t u = 0; u < kSegWords; ++u) raw[u] = C[min(w0 + u, T2w - 1u)];
   uint32_t tot = 0;
 #pragma unroll
   for (uint32_t u = 0; u < kSegWords; ++u) {
-    wv[u] = 0u;
-    if (u < nseg) {
-      const uint32_t wi = w0 + u;
-      wv[u] = wi < T2w ? C[wi] : 0u;
-      tot += (wv[u] & 0xFFFFu) + (wv[u] >> 16);
-    }
+    wv[u] = (u < nseg && w0 + u < T2w) ? raw[u] : 0u;
+    tot += (wv[u] & 0xFFFFu) + (wv[u] >> 16);
   }
   return tot;
 }
 
-/* Overlap k, column size n, FP32 first-term log x and class of each cell of
- * a segment (class 0 sum, 1 p == 1, 2 p == 0, 3 no cell). */
+/* The cells of one segment, straight-line: overlap k (prefix of the column
+ * counts), column size n (kept as 4n), and the FP32 log x of the first tail
+ * term,
+ *   x = (A_i + B_j) - (H[k] + H[K-k] + H[n-k] + H[N-K-n+k]),
+ * for sum-class cells (max(0, K+n-N) < k <= min(K, n)).  Cells with p == 1
+ * (k == 0, or k <= K+n-N, i.e. n-k >= N-K) or no column get x ~ +3e38;
+ * cells with k > K (p == 0: an empty tail; k <= n always, since k counts
+ * entries of the column's set) get x ~ -3e38, so that a segment minimum
+ * below the underflow bound flags every cell that may have p == 0.  Hb is
+ * the byte base of the H table; indices are byte offsets (4x), and the two
+ * that can wrap for cells of the other classes saturate at 0. */
 template <bool REGCOLS>
-__device__ __forceinline__ void segment_cells(const DevJob &J, const Lds &S, const float *H, const uint32_t *wv,
-                                              uint32_t nseg, uint32_t w0, uint32_t run, uint32_t K, float A,
-                                              const ColRegs &cr, uint32_t *k, uint32_t *nn, float *x,
-                                              uint32_t *cls) {
-  const uint32_t N = J.N, T2 = J.T2;
-  float b[kSegCells], h0[kSegCells], h1[kSegCells], h2[kSegCells], h3[kSegCells];
+__device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, const char *Hb, const uint32_t *wv,
+                                             uint32_t nseg, uint32_t w0, uint32_t run, uint32_t K, float A,
+                                             const ColRegs &cr, uint32_t *kk, uint32_t *nn4, float *xx) {
+  const uint32_t T2 = J.T2, K4 = K << 2, NK4 = (J.N - K) << 2;
+  float bc[kSegCells], h0[kSegCells], h1[kSegCells], h2[kSegCells], h3[kSegCells];
+  uint32_t d2[kSegCells];
+  /* indices and table loads of every cell first (one wait for all) */
 #pragma unroll
   for (uint32_t c = 0; c < kSegCells; ++c) {
     run += (c & 1u) ? (wv[c >> 1] >> 16) : (wv[c >> 1] & 0xFFFFu);
-    const uint32_t j = 2u * w0 + c;
-    const bool valid = (c >> 1) < nseg && j < T2;
-    k[c] = run;
+    const uint32_t k = run, k4 = k << 2;
+    uint32_t n4;
     if (REGCOLS) {
-      nn[c] = cr.n[c];
-      b[c] = cr.b[c];
+      n4 = cr.n4[c];
+      bc[c] = cr.b[c];
     } else {
-      const uint32_t jc = valid ? j : 0u;
-      nn[c] = S.coln[jc];
-      b[c] = S.colb[jc];
+      const uint32_t j = 2u * w0 + c;
+      const bool valid = (c >> 1) < nseg && j < T2;
+      const uint32_t jc = valid ? j : T2 - 1u;
+      n4 = S.coln[jc] << 2;
+      bc[c] = valid ? S.colb[jc] : kNoCell;
     }
-    const uint32_t n = nn[c], kc = k[c];
-    const uint32_t mn = (K + n > N) ? K + n - N : 0u;
-    const uint32_t mx = K < n ? K : n;
-    cls[c] = !valid ? 3u : (kc == 0u || kc <= mn) ? 1u : (kc > mx ? 2u : 0u);
-    const bool sum = cls[c] == 0u;
-    h0[c] = H[sum ? kc : 0u];
-    h1[c] = H[sum ? K - kc : 0u];
-    h2[c] = H[sum ? n - kc : 0u];
-    h3[c] = H[sum ? (N - K) - (n - kc) : 0u];
+    const uint32_t i1 = __builtin_elementwise_sub_sat(K4, k4); /* K - k */
+    const uint32_t i2 = n4 - k4;                                /* n - k */
+    const uint32_t i3 = __builtin_elementwise_sub_sat(NK4, i2); /* N - K - n + k */
+    kk[c] = k;
+    nn4[c] = n4;
+    d2[c] = i2;
+    h0[c] = *(const float *)(Hb + k4);
+    h1[c] = *(const float *)(Hb + i1);
+    h2[c] = *(const float *)(Hb + i2);
+    h3[c] = *(const float *)(Hb + i3);
   }
+  /* then each cell's class and screen value, one cell at a time so that
+   * only one cell's condition masks are live */
 #pragma unroll
   for (uint32_t c = 0; c < kSegCells; ++c) {
-    x[c] = (A + b[c]) - ((h0[c] + h1[c]) + (h2[c] + h3[c]));
-    if (cls[c] == 0u && x[c] + J.margin < -745.2f) {
-      /* every term from k on is smaller (ratio < 1) and its exponent is below
-       * the exp underflow point -745.1332...: the reference's sum is exactly 0 */
-      const uint32_t kc = k[c], n = nn[c];
-      const uint64_t num = (uint64_t)(K - kc) * (uint64_t)(n - kc);
-      const uint64_t den = (uint64_t)(kc + 1) * (uint64_t)((N - K) - (n - kc) + 1);
-      if (num < den) cls[c] = 2u;
+    __builtin_amdgcn_sched_barrier(0);
+    const bool one = kk[c] == 0u || d2[c] >= NK4;
+    const bool zero = kk[c] > K;
+    const float b = zero ? -kNoCell : (one ? kNoCell : bc[c]);
+    xx[c] = (A + b) - ((h0[c] + h1[c]) + (h2[c] + h3[c]));
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+/* True when a cell's p is exactly 0.0 in the reference: k > min(K, n) (the
+ * tail is empty), or every term from k on is below the exp underflow point
+ * (-745.1332...): x + margin < -745.2 bounds the first exponent and a ratio
+ * below 1 makes every later term smaller. */
+__device__ __forceinline__ bool exact_zero(const DevJob &J, uint32_t K, uint32_t n, uint32_t k, float x) {
+  const uint32_t N = J.N;
+  if (k > min(K, n)) return true;
+  if (!(x + J.margin < -745.2f)) return false;
+  const uint64_t num = (uint64_t)(K - k) * (uint64_t)(n - k);
+  const uint64_t den = (uint64_t)(k + 1) * (uint64_t)((N - K) - (n - k) + 1);
+  return num < den;
+}
+
+/* The p == 0 cell order (optimize_main.rs:93-116 at equal p): larger k, then
+ * smaller (i, j). */
+struct ZeroKey {
+  uint32_t k, ij;
+};
+__device__ __forceinline__ void take_zero(ZeroKey &z, uint32_t k, uint32_t ij) {
+  if (z.ij == kNone || k > z.k || (k == z.k && ij < z.ij)) {
+    z.k = k;
+    z.ij = ij;
+  }
+}
+__device__ __forceinline__ ZeroKey wave_zero(ZeroKey z) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint32_t k = __shfl_xor(z.k, off), ij = __shfl_xor(z.ij, off);
+    if (ij != kNone) take_zero(z, k, ij);
+  }
+  return z;
+}
+
+/* Exact best among the p == 1 cells (k <= max(0, K+n-N)): larger k, then
+ * smaller (i, j).  Only needed when no cell has p < 1; rebuilds the wave's
+ * column counts row by row (they are empty after the reverse sweep). */
+template <int NT>
+__device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout &Lo, const Lds &S, uint32_t *C) {
+  constexpr uint32_t W = NT / 64;
+  const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t T1 = J.T1, T2 = J.T2, T2w = J.T2w, N = J.N;
+  const uint32_t c2 = (T2w + 63) / 64, wbeg = lane * c2;
+  ZeroKey o = {0u, kNone};
+  uint32_t added = 0;
+  for (uint32_t i = w; i < T1; i += W) {
+    rows_update(S, C, added, i + 1, true);
+    added = i + 1;
+    const uint32_t K = J.K[i];
+    uint32_t run = row_lane_offset(C, wbeg, c2, T2w);
+    for (uint32_t t = 0; t < c2; ++t) {
+      const uint32_t wi = wbeg + t;
+      const uint32_t x = wi < T2w ? C[wi] : 0u;
+      for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t j = 2 * wi + h;
+        run += h ? (x >> 16) : (x & 0xFFFFu);
+        if (j >= T2) continue;
+        const uint32_t n = S.coln[j];
+        const uint32_t mn = (K + n > N) ? K + n - N : 0u;
+        if (run <= mn) take_zero(o, run, (i << 16) | j);
+      }
     }
   }
+  return wave_zero(o);
 }
 
 /* One permutation: overlap grid -> best cell.  Returns the best key in
@@ -660,35 +731,15 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   uint32_t *C = S.cw + w * Lo.T2w_pad;
   const uint32_t c2 = (T2w + 63) / 64;
   const uint32_t wbeg = lane * c2;
-  const float *H = HLDS ? S.h : J.H32;
-  /* this lane's column constants, kept in registers across rows */
-  ColRegs cr;
-  if (REGCOLS && !PLAIN) {
-#pragma unroll
-    for (uint32_t c = 0; c < kSegCells; ++c) {
-      const uint32_t j = 2u * wbeg + c;
-      const uint32_t jc = (c >> 1) < c2 && j < T2 ? j : 0u;
-      cr.n[c] = S.coln[jc];
-      cr.b[c] = S.colb[jc];
-    }
-  }
+  const char *Hb = HLDS ? (const char *)S.h : (const char *)J.H32;
 
-  BestKey zb = {0.0, 0u, kNone};            /* p == 0 class */
-  BestKey ob = {1.0, 0u, kNone};            /* p == 1 class */
-  BestKey xb = {__builtin_inf(), 0u, kNone}; /* exactly evaluated cells */
-  float Urun = __builtin_inff();
-  /* this lane's cell with the smallest first-term log: its tail bracket,
-   * taken after the sweep, bounds the minimum from above */
-  float bx = __builtin_inff();
-  uint32_t bK = 0, bn = 0, bk = 0;
-
-  /* ---- D: forward sweep over this wave's rows w, w+W, ... ---- */
-  uint32_t added = 0; /* rows [0, added) are in C */
-  for (uint32_t i = w; i < T1; i += W) {
-    rows_update(S, C, added, i + 1, true);
-    added = i + 1;
-    const uint32_t K = J.K[i];
-    if (PLAIN) {
+  if (PLAIN) {
+    BestKey xb = {__builtin_inf(), 0u, kNone};
+    uint32_t added = 0;
+    for (uint32_t i = w; i < T1; i += W) {
+      rows_update(S, C, added, i + 1, true);
+      added = i + 1;
+      const uint32_t K = J.K[i];
       uint32_t run = row_lane_offset(C, wbeg, c2, T2w);
 #pragma unroll 1
       for (uint32_t t = 0; t < c2; ++t) {
@@ -704,47 +755,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
           take_better(xb, p, k, (i << 16) | j);
         }
       }
-      continue;
     }
-    const float A = J.A32[i];
-    float lmin = __builtin_inff();
-    uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
-#pragma unroll 1
-    for (uint32_t t0 = 0; t0 < c2; t0 += kSegWords) {
-      const uint32_t nseg = c2 - t0 < kSegWords ? c2 - t0 : kSegWords;
-      uint32_t wv[kSegWords];
-      const uint32_t tot = read_segment(C, wbeg + t0, nseg, T2w, wv);
-      if (REGCOLS) run = wave_excl_scan(tot);
-      uint32_t kk[kSegCells], nn[kSegCells], cls[kSegCells];
-      float xx[kSegCells];
-      segment_cells<REGCOLS>(J, S, H, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn, xx, cls);
-      run += tot;
-#pragma unroll
-      for (uint32_t c = 0; c < kSegCells; ++c) {
-        const uint32_t ij = (i << 16) | (2u * (wbeg + t0) + c);
-        if (cls[c] == 1u) {
-          take_better(ob, 1.0, kk[c], ij);
-          Urun = fminf(Urun, 0.f);
-        } else if (cls[c] == 2u) {
-          take_better(zb, 0.0, kk[c], ij);
-          Urun = -__builtin_inff();
-        } else if (cls[c] == 0u) {
-          const float L = xx[c] - J.margin;
-          lmin = fminf(lmin, L < -708.f ? -__builtin_inff() : L);
-          if (xx[c] < bx) {
-            bx = xx[c];
-            bK = K;
-            bn = nn[c];
-            bk = kk[c];
-          }
-        }
-      }
-    }
-    lmin = wave_minf(lmin);
-    if (lane == 0) S.rowmin[i] = lmin;
-  }
-
-  if (PLAIN) {
     xb = wave_best(xb);
     if (lane == 0) S.redB[2 * W + w] = xb;
     __syncthreads();
@@ -755,16 +766,96 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     return b;
   }
 
-  if (bx < __builtin_inff()) Urun = fminf(Urun, screen_upper(J, bK, bn, bk, bx));
+  /* this lane's column constants, kept in registers across rows; columns
+   * past T2 repeat the last column's n (their k equals its k) and get no cell */
+  ColRegs cr;
+  if (REGCOLS) {
+#pragma unroll
+    for (uint32_t c = 0; c < kSegCells; ++c) {
+      const uint32_t j = 2u * wbeg + c;
+      const bool valid = (c >> 1) < c2 && j < T2;
+      const uint32_t jc = valid ? j : T2 - 1u;
+      cr.n4[c] = S.coln[jc] << 2;
+      cr.b[c] = valid ? S.colb[jc] : kNoCell;
+    }
+  }
+  const float margin = J.margin;
+  const float uthr = -745.2f - margin; /* below: the first term may underflow */
+
+  ZeroKey zb = {0u, kNone};          /* p == 0 cells */
+  BestKey xb = {__builtin_inf(), 0u, kNone}; /* exactly evaluated cells */
+  /* this lane's cell with the smallest screen value: its tail bracket, taken
+   * after the sweep, bounds the minimum log p from above */
+  float bx = __builtin_inff();
+  uint32_t bK = 0, bn = 0, bk = 0;
+
+  /* ---- D: forward sweep over this wave's rows w, w+W, ... ---- */
+  uint32_t added = 0; /* rows [0, added) are in C */
+  for (uint32_t i = w; i < T1; i += W) {
+    rows_update(S, C, added, i + 1, true);
+    added = i + 1;
+    const uint32_t K = J.K[i];
+    const float A = J.A32[i];
+    float lmin = __builtin_inff(); /* packed (value, cell) */
+    uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
+    /* REGCOLS: the row is one segment */
+#pragma unroll 1
+    for (uint32_t t0 = 0; REGCOLS ? t0 == 0u : t0 < c2; t0 += kSegWords) {
+      const uint32_t nseg = REGCOLS ? c2 : (c2 - t0 < kSegWords ? c2 - t0 : kSegWords);
+      uint32_t wv[kSegWords];
+      const uint32_t tot = read_segment(C, wbeg + t0, nseg, T2w, wv);
+      if (REGCOLS) run = wave_excl_scan(tot);
+      uint32_t kk[kSegCells], nn4[kSegCells];
+      float xx[kSegCells];
+      segment_fast<REGCOLS>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+      run += tot;
+      float smin = __builtin_inff();
+#pragma unroll
+      for (uint32_t c = 0; c < kSegCells; ++c) smin = fminf(smin, pack_cell(xx[c], c));
+      const bool rare = smin < uthr + pack_slack(smin);
+      if (smin < bx) {
+        /* this lane's best so far is in this segment */
+        const uint32_t c = packed_cell(smin);
+        uint32_t k = kk[0], n4 = nn4[0];
+#pragma unroll
+        for (uint32_t u = 1; u < kSegCells; ++u) {
+          k = c == u ? kk[u] : k;
+          n4 = c == u ? nn4[u] : n4;
+        }
+        bx = smin;
+        bK = K;
+        bn = n4 >> 2;
+        bk = k;
+      }
+      lmin = fminf(lmin, smin);
+      if (__ballot(rare)) {
+        /* exact p == 0 cells of this segment */
+#pragma unroll
+        for (uint32_t c = 0; c < kSegCells; ++c) {
+          const uint32_t j = 2u * (wbeg + t0) + c;
+          if ((c >> 1) < nseg && j < T2 && exact_zero(J, K, nn4[c] >> 2, kk[c], xx[c]))
+            take_zero(zb, kk[c], (i << 16) | j);
+        }
+      }
+    }
+    lmin = wave_minf(lmin);
+    if (lane == 0) {
+      /* lower bound of the row's sum-class log p (subnormal sums: -inf) */
+      const float L = lmin - margin - pack_slack(lmin);
+      S.rowmin[i] = L < -708.f ? -__builtin_inff() : L;
+    }
+  }
+
+  float Urun = 1e-6f; /* every p <= 1 (+ rounding): log p <= 1e-6 always */
+  if (bx < 1e30f) Urun = fminf(Urun, screen_upper(J, bK, bn, bk, bx + pack_slack(bx)));
   Urun = wave_minf(Urun);
-  zb = wave_best(zb);
-  ob = wave_best(ob);
+  zb = wave_zero(zb);
+  if (zb.ij != kNone) Urun = -__builtin_inff();
   uint64_t tc2 = cycles();
   st.cyc[1] += tc2 - tc1;
   if (lane == 0) {
     S.redU[w] = Urun;
-    S.redB[w] = zb;
-    S.redB[W + w] = ob;
+    S.redB[w] = BestKey{0.0, zb.k, zb.ij};
   }
   __syncthreads();
   float Ustar = __builtin_inff();
@@ -794,33 +885,47 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
         again = false;
         uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
 #pragma unroll 1
-        for (uint32_t t0 = 0; t0 < c2 && !again; t0 += kSegWords) {
-          const uint32_t nseg = c2 - t0 < kSegWords ? c2 - t0 : kSegWords;
+        for (uint32_t t0 = 0; (REGCOLS ? t0 == 0u : t0 < c2) && !again; t0 += kSegWords) {
+          const uint32_t nseg = REGCOLS ? c2 : (c2 - t0 < kSegWords ? c2 - t0 : kSegWords);
           uint32_t wv[kSegWords];
           const uint32_t tot = read_segment(C, wbeg + t0, nseg, T2w, wv);
           if (REGCOLS) run = wave_excl_scan(tot);
-          uint32_t kk[kSegCells], nn[kSegCells], cls[kSegCells];
+          uint32_t kk[kSegCells], nn4[kSegCells];
           float xx[kSegCells];
-          segment_cells<REGCOLS>(J, S, H, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn, xx, cls);
+          segment_fast<REGCOLS>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
           run += tot;
+          /* cheap first test: the first-term bound alone; cells whose bound
+           * is below -708 (possibly subnormal or zero sums) always pass */
+          const float thr = fmaxf(Uw + margin, -708.f + margin);
+          uint32_t surv = 0;
+#pragma unroll
+          for (uint32_t c = 0; c < kSegCells; ++c) surv |= (xx[c] <= thr && 2u * t0 + c >= resume) ? 1u << c : 0u;
+          if (!__ballot(surv != 0u)) continue;
 #pragma unroll
           for (uint32_t c = 0; c < kSegCells; ++c) {
-            const uint32_t ord = 2u * t0 + c;
             bool f = false;
-            if (cls[c] == 0u && ord >= resume && !again) {
-              float L = xx[c] - J.margin;
-              if (L <= Uw && L >= -708.f) L += tail_lower_log(J.N, K, nn[c], kk[c]);
-              L = L < -708.f ? -__builtin_inff() : L;
-              f = L <= Uw;
+            if (surv & (1u << c)) {
+              /* p == 0 cells are already in the zero class */
+              if (!(xx[c] < uthr && exact_zero(J, K, nn4[c] >> 2, kk[c], xx[c]))) {
+                float L = xx[c] - margin;
+                if (L >= -708.f) {
+                  L += tail_lower_log(J.N, K, nn4[c] >> 2, kk[c]);
+                  f = L <= Uw;
+                } else {
+                  f = true; /* possibly subnormal: no usable lower bound */
+                }
+              }
             }
+            f = f && !again;
             const uint64_t m = __ballot(f);
             const uint32_t cnt = (uint32_t)__popcll(m);
             if (nc + cnt > (uint32_t)kCandCap) {
               again = true;
-              resume = ord;
+              resume = 2u * t0 + c;
             } else if (cnt) {
               if (f) cand[nc + mask_lt(m)] = make_uint2((i << 16) | (2u * (wbeg + t0) + c), kk[c]);
               nc += cnt;
+              st.push1 += cnt;
             }
           }
         }
@@ -848,9 +953,24 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   st.cyc[4] += cycles() - tc5;
   if (lane == 0) S.redB[2 * W + w] = xb;
   __syncthreads();
+  /* p == 1 cells matter only when no cell has p < 1 */
+  bool need1 = true;
+  for (uint32_t v = 0; v < W; ++v) {
+    if (S.redB[v].ij != kNone) need1 = false;                                    /* a p == 0 cell */
+    if (S.redB[2 * W + v].ij != kNone && S.redB[2 * W + v].p < 1.0) need1 = false; /* a p < 1 cell */
+  }
+  if (need1) {
+    const ZeroKey o = class1_best<NT>(J, Lo, S, C);
+    if (lane == 0) S.redB[W + w] = BestKey{1.0, o.k, o.ij};
+  } else if (lane == 0) {
+    S.redB[W + w] = BestKey{1.0, 0u, kNone};
+  }
+  __syncthreads();
   BestKey b = {__builtin_inf(), 0u, kNone};
-  if (tid == 0)
+  if (tid == 0) {
     for (uint32_t v = 0; v < 3 * W; ++v) take_better(b, S.redB[v].p, S.redB[v].k, S.redB[v].ij);
+    if (b.p > 0.0 && b.p < 1.0 && Ustar < 1e30f) st.gap += (uint32_t)fmaxf(0.f, (Ustar - (float)log(b.p)) * 1000.f);
+  }
   __syncthreads();
   return b;
 }
@@ -865,6 +985,10 @@ __device__ __forceinline__ void flush_stats(const DevJob &J, const WaveStats &st
     if (st.rows) atomicAdd(&J.stats[2], (unsigned long long)st.rows);
     for (int c = 0; c < 5; ++c)
       if (st.cyc[c]) atomicAdd(&J.stats[3 + c], (unsigned long long)st.cyc[c]);
+    if (st.push1) atomicAdd(&J.stats[8], (unsigned long long)st.push1);
+    if (st.push2) atomicAdd(&J.stats[9], (unsigned long long)st.push2);
+    if (st.flushes) atomicAdd(&J.stats[10], (unsigned long long)st.flushes);
+    if (st.gap) atomicAdd(&J.stats[11], (unsigned long long)st.gap);
   }
 }
 
@@ -910,7 +1034,7 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
   load_job_tables<NT, HLDS>(J, S);
   __syncthreads();
   uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * J.N2;
-  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}};
+  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u};
   for (uint64_t pp = blockIdx.x; pp < n_perm; pp += gridDim.x) {
     const uint64_t pid = perm_base + pp;
     const uint32_t *i1 = (EXPLICIT && idx1) ? idx1 + pp * J.N1 : nullptr;
@@ -967,7 +1091,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs,
   __shared__ uint32_t s_item;
   uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * ws_stride;
   uint32_t cur = kNone;
-  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}};
+  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u};
   const DevJob *last = nullptr;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
@@ -983,7 +1107,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs,
     const Lds S = lds_view(smem, B.lo, W);
     if (wi.job != cur) {
       if (last) flush_stats(*last, st);
-      st = {0u, 0u, 0u, {0, 0, 0, 0, 0}};
+      st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u};
       last = &B.J;
       load_job_tables<NT, HLDS>(B.J, S);
       __syncthreads();
