@@ -403,8 +403,7 @@ int dto_internal_shuffle_device(uint64_t seed, uint32_t job, uint32_t perm, uint
   if ((rc = current_device(&dev))) return rc;
   DevBuf d;
   HIP_TRY(hipMalloc(&d.p, std::max<uint32_t>(n, 1) * 4));
-  uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 1024));
-  hipLaunchKernelGGL(k_shuffle, dim3(g), dim3(256), 0, 0, seed, job, perm, list, n, (uint32_t *)d.p);
+  hipLaunchKernelGGL(k_shuffle, dim3(1), dim3(256), 0, 0, seed, job, perm, list, n, (uint32_t *)d.p);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpy(out, d.p, (size_t)n * 4, hipMemcpyDeviceToHost));
   return DTO_OK;
@@ -425,7 +424,7 @@ int dto_internal_stats(dto_job *job, uint64_t *out, int reset) {
 int dto_philox_permutation(uint64_t seed, uint32_t job_id, uint32_t perm, uint32_t list, uint32_t len, uint32_t *out) {
   if (!out && len) return fail(DTO_ERR_INVALID_ARGUMENT, "null output");
   Shuffle s = shuffle_make(seed, job_id, perm, list, len);
-  for (uint32_t k = 0; k < len; ++k) out[k] = shuffle_index(s, k);
+  shuffle_fill(s, out);
   return DTO_OK;
 }
 

@@ -198,6 +198,45 @@ __device__ __forceinline__ void load_job_tables(const DevJob &J, const Lds &S) {
   }
 }
 
+/* Visits the Philox permutation of one list in rank order, fn(k, idx[k])
+ * for every k < n, spread over the workgroup.  The domain [0, 2^b) goes in
+ * chunks of NT * 8 consecutive elements, each thread evaluating 8
+ * consecutive ones; a workgroup prefix count (wave scan + per-wave totals in
+ * wsum[W]) ranks those that fall below n.  Every thread runs every chunk. */
+template <int NT, typename Fn>
+__device__ __forceinline__ void shuffle_visit(const Shuffle &s, uint32_t *wsum, Fn fn) {
+  constexpr uint32_t kU = 8, W = NT / 64;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  uint32_t carried = 0;
+  for (uint32_t base = 0; base < s.dom; base += NT * kU) {
+    uint32_t v[kU], cnt = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      const uint32_t p = base + tid * kU + u;
+      v[u] = p < s.dom ? feistel_once(s, p) : s.n;
+      cnt += v[u] < s.n ? 1u : 0u;
+    }
+    const uint32_t ex = wave_excl_scan(cnt);
+    if (lane == 63) wsum[w] = ex + cnt;
+    __syncthreads();
+    uint32_t r = carried + ex, total = 0;
+    for (uint32_t x = 0; x < W; ++x) {
+      const uint32_t t = wsum[x];
+      r += x < w ? t : 0u;
+      total += t;
+    }
+    __syncthreads();
+    carried += total;
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      if (v[u] < s.n) {
+        fn(r, v[u]);
+        ++r;
+      }
+    }
+  }
+}
+
 /* Phases A-B for one permutation: leaves the row-sorted list2 bins in S.nb,
  * row offsets in S.rowstart and zeroed per-wave column counts.  The (a, b) pairs are
  * staged in `pairs` (LDS: the screen-table region, reloaded afterwards when
@@ -216,71 +255,85 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
   /* A: list1 feature m sits at the position k with idx1[k] == m (loads of
    * kU positions in flight per thread) */
   constexpr uint32_t kU = 8;
-  for (uint32_t k0 = tid; k0 < J.N1; k0 += NT * kU) {
-    uint32_t m[kU], bn[kU];
+  if (!EXPLICIT) {
+    shuffle_visit<NT>(s1, S.redS, [&](uint32_t k, uint32_t m) { S.nb[m] = (uint16_t)J.bin1[k]; });
+  } else {
+    for (uint32_t k0 = tid; k0 < J.N1; k0 += NT * kU) {
+      uint32_t m[kU], bn[kU];
 #pragma unroll
-    for (uint32_t u = 0; u < kU; ++u) {
-      const uint32_t k = k0 + u * NT;
-      m[u] = 0u;
-      bn[u] = 0u;
-      if (k < J.N1) {
-        m[u] = EXPLICIT ? (i1 ? i1[k] : k) : shuffle_index(s1, k);
-        bn[u] = J.bin1[k];
+      for (uint32_t u = 0; u < kU; ++u) {
+        const uint32_t k = k0 + u * NT;
+        m[u] = 0u;
+        bn[u] = 0u;
+        if (k < J.N1) {
+          m[u] = i1 ? i1[k] : k;
+          bn[u] = J.bin1[k];
+        }
       }
-    }
 #pragma unroll
-    for (uint32_t u = 0; u < kU; ++u) {
-      if (k0 + u * NT < J.N1) {
-        if (m[u] < J.N1)
-          S.nb[m[u]] = (uint16_t)bn[u];
-        else
-          atomicOr(err, 1);
+      for (uint32_t u = 0; u < kU; ++u) {
+        if (k0 + u * NT < J.N1) {
+          if (m[u] < J.N1)
+            S.nb[m[u]] = (uint16_t)bn[u];
+          else
+            atomicOr(err, 1);
+        }
       }
     }
   }
   __syncthreads();
   /* B1: (a, b) per list2 position, counted per row a */
-  for (uint32_t e0 = tid; e0 < J.N2; e0 += NT * kU) {
-    uint32_t q[kU], b[kU];
-    int32_t jm[kU];
-#pragma unroll
-    for (uint32_t u = 0; u < kU; ++u) {
-      const uint32_t e = e0 + u * NT;
-      q[u] = kNone;
-      b[u] = kBinNone;
-      if (e < J.N2) {
-        q[u] = EXPLICIT ? (i2 ? i2[e] : e) : shuffle_index(s2, e);
-        b[u] = J.bin2[e];
+  auto pair_of = [&](uint32_t e, uint32_t q, uint32_t b, int32_t jm) {
+    uint32_t a = kBinNone;
+    if (!J.dup) {
+      if (jm >= 0) a = S.nb[jm];
+    } else {
+      /* intersect_genes collapses duplicate ids of genes1: the entry is in
+       * S1'(t) as soon as any list1 entry with its id is */
+      for (uint32_t t = J.join_off[q]; t < J.join_off[q + 1]; ++t) {
+        uint32_t v = S.nb[J.join_pos[t]];
+        a = v < a ? v : a;
       }
     }
+    uint32_t pr = kNone;
+    if (a < T1 && b < T2) {
+      pr = (a << 16) | b;
+      atomicAdd(&S.rowstart[a], 1u);
+    }
+    if (PL)
+      lpairs[e] = pr;
+    else
+      vpairs[e] = pr;
+  };
+  if (!EXPLICIT) {
+    shuffle_visit<NT>(s2, S.redS, [&](uint32_t e, uint32_t q) {
+      pair_of(e, q, J.bin2[e], J.dup ? -1 : J.join[q]);
+    });
+  } else {
+    for (uint32_t e0 = tid; e0 < J.N2; e0 += NT * kU) {
+      uint32_t q[kU], b[kU];
+      int32_t jm[kU];
 #pragma unroll
-    for (uint32_t u = 0; u < kU; ++u) jm[u] = (q[u] < J.N2 && !J.dup) ? J.join[q[u]] : -1;
-#pragma unroll
-    for (uint32_t u = 0; u < kU; ++u) {
-      const uint32_t e = e0 + u * NT;
-      if (e >= J.N2) continue;
-      uint32_t a = kBinNone;
-      if (q[u] >= J.N2) {
-        atomicOr(err, 1);
-      } else if (!J.dup) {
-        if (jm[u] >= 0) a = S.nb[jm[u]];
-      } else {
-        /* intersect_genes collapses duplicate ids of genes1: the entry is in
-         * S1'(t) as soon as any list1 entry with its id is */
-        for (uint32_t t = J.join_off[q[u]]; t < J.join_off[q[u] + 1]; ++t) {
-          uint32_t v = S.nb[J.join_pos[t]];
-          a = v < a ? v : a;
+      for (uint32_t u = 0; u < kU; ++u) {
+        const uint32_t e = e0 + u * NT;
+        q[u] = kNone;
+        b[u] = kBinNone;
+        if (e < J.N2) {
+          q[u] = i2 ? i2[e] : e;
+          b[u] = J.bin2[e];
         }
       }
-      uint32_t pr = kNone;
-      if (a < T1 && b[u] < T2) {
-        pr = (a << 16) | b[u];
-        atomicAdd(&S.rowstart[a], 1u);
+#pragma unroll
+      for (uint32_t u = 0; u < kU; ++u) jm[u] = (q[u] < J.N2 && !J.dup) ? J.join[q[u]] : -1;
+#pragma unroll
+      for (uint32_t u = 0; u < kU; ++u) {
+        const uint32_t e = e0 + u * NT;
+        if (e >= J.N2) continue;
+        if (q[u] >= J.N2)
+          atomicOr(err, 1);
+        else
+          pair_of(e, q[u], b[u], jm[u]);
       }
-      if (PL)
-        lpairs[e] = pr;
-      else
-        vpairs[e] = pr;
     }
   }
   if (!PL) __threadfence_block();
@@ -1200,11 +1253,13 @@ static __global__ __launch_bounds__(256) void k_exp(const double *__restrict__ x
     out[c] = dto_exp(x[c], tab);
 }
 
-/* Philox shuffle on the device (checks that host and device draw the same permutation). */
+/* Philox shuffle on the device (checks that host and device draw the same
+ * permutation); one workgroup of 256 threads. */
 static __global__ __launch_bounds__(256) void k_shuffle(uint64_t seed, uint32_t job, uint32_t perm, uint32_t list,
                                                  uint32_t n, uint32_t *__restrict__ out) {
-  Shuffle s = shuffle_make(seed, job, perm, list, n);
-  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) out[k] = shuffle_index(s, k);
+  __shared__ uint32_t wsum[4];
+  const Shuffle s = shuffle_make(seed, job, perm, list, n);
+  shuffle_visit<256>(s, wsum, [&](uint32_t k, uint32_t v) { out[k] = v; });
 }
 
 }  // namespace dto

@@ -224,52 +224,63 @@ DTO_HD U32x4 philox4x32_10(U32x4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
-/* A keyed bijection of [0, n): an alternating Feistel network over
- * b = ceil(log2 n) bits (left half ceil(b/2), right half floor(b/2) bits;
- * each round XORs a keyed mix of one half into the other, so every round and
- * hence the whole network is invertible), applied repeatedly until the value
- * falls inside [0, n) (cycle walking keeps it a bijection of [0, n)).  The 12
- * round keys come from one Philox4x32-10 block keyed by the seed, with the
- * (job, permutation, list) triple as the counter. */
-#define DTO_FEISTEL_ROUNDS 12
+/* A keyed permutation of [0, n).  F is an alternating Feistel network over
+ * the domain [0, 2^b), b = max(12, ceil(log2 n)) (left half ceil(b/2) bits,
+ * right half floor(b/2) bits); each round XORs a keyed mix of one half into
+ * the other, so every round, and F, is a bijection of the domain.  The
+ * permutation is F's image of the domain in order, keeping the values < n:
+ *   idx = [F(p) for p = 0, 1, ..., 2^b - 1 if F(p) < n],
+ * which holds every value of [0, n) exactly once; the 12-bit floor keeps
+ * short lists well mixed (their order is that of their values' preimages in
+ * a 4096-element permutation).  (Every domain element is
+ * evaluated once, so a GPU workgroup computes it without divergence, with a
+ * prefix count for the positions.)  The 8 round keys come from two
+ * Philox4x32-10 blocks keyed by the seed, with (permutation, job, list) as
+ * the counter.  The mix uses 24-bit multiplies (full rate on the GPU). */
+#define DTO_FEISTEL_ROUNDS 8
 struct Shuffle {
-  uint32_t n, bits_l, bits_r, mask_l, mask_r;
+  uint32_t n, bits_l, bits_r, mask_l, mask_r, dom;
   uint32_t rk[DTO_FEISTEL_ROUNDS];
 };
 
 DTO_HD Shuffle shuffle_make(uint64_t seed, uint32_t job, uint32_t perm, uint32_t list, uint32_t n) {
   Shuffle s;
   s.n = n;
-  uint32_t b = 1;
-  while (b < 32 && (1u << b) < n) ++b;
-  if (b < 2) b = 2;
+  uint32_t b = 12;
+  while (b < 31 && (1u << b) < n) ++b;
   s.bits_l = (b + 1) / 2;
   s.bits_r = b / 2;
   s.mask_l = (1u << s.bits_l) - 1;
   s.mask_r = (1u << s.bits_r) - 1;
-  U32x4 c = {perm, job, list, 0x44544F31u};
+  s.dom = 1u << b;
+  U32x4 c = {perm, job, list, 0x44544F32u};
   U32x4 a = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
   c.w ^= 0x5A5A5A5Au;
   U32x4 bb = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  c.w ^= 0x0F0F0F0Fu;
-  U32x4 cc = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  uint32_t w[12] = {a.x, a.y, a.z, a.w, bb.x, bb.y, bb.z, bb.w, cc.x, cc.y, cc.z, cc.w};
-  for (int r = 0; r < DTO_FEISTEL_ROUNDS; ++r) s.rk[r] = w[r];
+  s.rk[0] = a.x;
+  s.rk[1] = a.y;
+  s.rk[2] = a.z;
+  s.rk[3] = a.w;
+  s.rk[4] = bb.x;
+  s.rk[5] = bb.y;
+  s.rk[6] = bb.z;
+  s.rk[7] = bb.w;
   return s;
 }
 
+/* Round function: two multiply-xorshift steps on 24-bit operands. */
 DTO_HD uint32_t feistel_mix(uint32_t v, uint32_t key) {
-  uint32_t h = (v ^ key) * 0xCD9E8D57u;
+  uint32_t h = ((v ^ key) & 0xFFFFFFu) * 0x9E3779u;
   h ^= h >> 15;
-  h *= 0xD2511F53u;
+  h = (h & 0xFFFFFFu) * 0x85EBCBu;
   h ^= h >> 13;
-  h *= 0x9E3779B1u;
-  h ^= h >> 16;
-  return h;
+  return h ^ (key >> 24);
 }
 
+/* F(p): the Feistel bijection of [0, 2^b). */
 DTO_HD uint32_t feistel_once(const Shuffle &s, uint32_t v) {
   uint32_t L = v >> s.bits_r, R = v & s.mask_r;
+#pragma unroll
   for (int r = 0; r < DTO_FEISTEL_ROUNDS; ++r) {
     if ((r & 1) == 0)
       L ^= feistel_mix(R, s.rk[r]) & s.mask_l;
@@ -279,13 +290,13 @@ DTO_HD uint32_t feistel_once(const Shuffle &s, uint32_t v) {
   return (L << s.bits_r) | R;
 }
 
-/* idx[k]: the sorted position whose feature lands at position k. */
-DTO_HD uint32_t shuffle_index(const Shuffle &s, uint32_t k) {
-  uint32_t v = feistel_once(s, k);
-  /* the cycle through k returns to k < n, so this loop ends; the domain is
-   * < 4n, so it takes < 4 steps on average */
-  while (v >= s.n) v = feistel_once(s, v);
-  return v;
+/* The whole permutation (host): out[k] = idx[k], k < n. */
+static inline void shuffle_fill(const Shuffle &s, uint32_t *out) {
+  uint32_t k = 0;
+  for (uint32_t p = 0; p < s.dom; ++p) {
+    const uint32_t v = feistel_once(s, p);
+    if (v < s.n) out[k++] = v;
+  }
 }
 
 }  // namespace dto
