@@ -75,7 +75,9 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
   };
   region(L.nb, 2u * std::max(std::max(N1, N2), 1u));
   region(L.rowstart, 4u * (T1 + 1));
-  region(L.cursor, 4u * T1);
+  /* cursor is used while building the rows, cand0 in the reverse sweep */
+  region(L.cursor, std::max(4u * T1, 8u * (uint32_t)W * (uint32_t)kCand0Cap));
+  L.cand0 = L.cursor;
   region(L.cw, 4u * (uint32_t)W * L.T2w_pad);
   region(L.rowmin, 4u * T1);
   region(L.coln, 4u * T2);
@@ -101,6 +103,7 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
   for (uint32_t *f : {&L.nb, &L.rowstart, &L.cursor, &L.cw, &L.rowmin, &L.coln, &L.colb, &L.cand, &L.cand3,
                       &L.exptab, &L.red})
     *f += base;
+  L.cand0 = L.cursor;
   L.total = base + fixed;
   return L;
 }
@@ -126,7 +129,7 @@ int upload_job(dto_job *j) {
   size_t o_t1 = put(buf, h.thr1), o_t2 = put(buf, h.thr2), o_LF = put(buf, h.LF), o_D = put(buf, h.D);
   size_t o_H = put(buf, h.H32), o_A = put(buf, h.A32), o_B = put(buf, h.B32);
   size_t o_err = put(buf, std::vector<int>(4, 0));
-  size_t o_stats = put(buf, std::vector<unsigned long long>(12, 0));
+  size_t o_stats = put(buf, std::vector<unsigned long long>(16, 0));
   if (j->arena) {
     (void)hipFree(j->arena);
     j->arena = nullptr;
@@ -410,14 +413,15 @@ int dto_internal_shuffle_device(uint64_t seed, uint32_t job, uint32_t perm, uint
 }
 
 /* work counters of a job's screened launches (test / profiling hook, not in
- * dto.h), 12 values: exact cells, exact terms, rescanned rows, cycles of the
+ * dto.h), 16 values: exact cells, exact terms, rescanned rows, cycles of the
  * build / forward / reverse / tier-1 flush / exact phases, tier-1 pushes,
- * tier-2 survivors, tier-1 flushes */
+ * tier-2 survivors, tier-1 flushes, first-bound gap (milli-nats), cells past
+ * the reverse sweep's first-term test */
 int dto_internal_stats(dto_job *job, uint64_t *out, int reset) {
   if (!job || !out) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
   HIP_TRY(hipDeviceSynchronize());
-  HIP_TRY(hipMemcpy(out, job->dev.stats, 12 * sizeof(uint64_t), hipMemcpyDeviceToHost));
-  if (reset) HIP_TRY(hipMemset(job->dev.stats, 0, 12 * sizeof(uint64_t)));
+  HIP_TRY(hipMemcpy(out, job->dev.stats, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if (reset) HIP_TRY(hipMemset(job->dev.stats, 0, 16 * sizeof(uint64_t)));
   return DTO_OK;
 }
 

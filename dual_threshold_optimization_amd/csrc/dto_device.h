@@ -35,7 +35,7 @@ struct DevJob {
   const float *H32;         /* [N+1] LF[x] - x*(ln N - 1), rounded to f32 */
   const float *A32;         /* [T1] H(K_i) + H(N-K_i) */
   const float *B32;         /* [T2] H(n_j) + H(N-n_j) - H(N) */
-  unsigned long long *stats; /* [12] work counters (dto_internal_stats) */
+  unsigned long long *stats; /* [16] work counters (dto_internal_stats) */
 };
 
 /* Byte offsets of the dynamic LDS regions of one workgroup. */
@@ -50,6 +50,7 @@ struct LdsLayout {
   uint32_t rowmin;   /* f32[T1] per-row minimum screen lower bound */
   uint32_t coln;     /* u32[T2] */
   uint32_t colb;     /* f32[T2] */
+  uint32_t cand0;    /* uint2[W][kCand0Cap] cells past the first-term test (aliases cursor) */
   uint32_t cand;     /* uint2[W][kCandCap] tier-1 survivors */
   uint32_t cand3;    /* uint4[W][kCand3Cap] tier-2 survivors */
   uint32_t exptab;   /* u64[256] */
@@ -59,6 +60,7 @@ struct LdsLayout {
   uint32_t h_in_lds;
 };
 
+constexpr int kCand0Cap = 128;
 constexpr int kCandCap = 64;
 constexpr int kCand3Cap = 32;
 

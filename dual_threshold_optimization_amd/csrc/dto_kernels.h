@@ -137,6 +137,7 @@ struct Lds {
   float *rowmin;
   uint32_t *coln;
   float *colb;
+  uint2 *cand0;
   uint2 *cand;
   uint4 *cand3;
   uint64_t *tab;
@@ -157,6 +158,7 @@ __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo
   s.rowmin = (float *)(smem + Lo.rowmin);
   s.coln = (uint32_t *)(smem + Lo.coln);
   s.colb = (float *)(smem + Lo.colb);
+  s.cand0 = (uint2 *)(smem + Lo.cand0);
   s.cand = (uint2 *)(smem + Lo.cand);
   s.cand3 = (uint4 *)(smem + Lo.cand3);
   s.tab = (uint64_t *)(smem + Lo.exptab);
@@ -492,6 +494,7 @@ struct WaveStats {
   uint64_t cyc[5]; /* build, forward sweep, reverse sweep, tier-1 flushes, exact */
   uint32_t push1, push2, flushes; /* tier-1 pushes, tier-2 survivors, tier-1 list flushes */
   uint32_t gap;                    /* sum of (first upper bound - log of the minimum), milli-nats */
+  uint32_t cheap;                  /* cells passing the first-term test of the reverse sweep (per lane) */
 };
 
 __device__ __forceinline__ uint64_t cycles() { return clock64(); }
@@ -591,6 +594,71 @@ __device__ __forceinline__ void flush_candidates(const DevJob &J, const Lds &S, 
   }
   nc = 0;
   st.flushes++;
+  wave_sync();
+}
+
+/* True when a cell's p is exactly 0.0 in the reference: k > min(K, n) (the
+ * tail is empty), or every term from k on is below the exp underflow point
+ * (-745.1332...): x + margin < -745.2 bounds the first exponent and a ratio
+ * below 1 makes every later term smaller. */
+__device__ __forceinline__ bool exact_zero(const DevJob &J, uint32_t K, uint32_t n, uint32_t k, float x) {
+  const uint32_t N = J.N;
+  if (k > min(K, n)) return true;
+  if (!(x + J.margin < -745.2f)) return false;
+  const uint64_t num = (uint64_t)(K - k) * (uint64_t)(n - k);
+  const uint64_t den = (uint64_t)(k + 1) * (uint64_t)((N - K) - (n - k) + 1);
+  return num < den;
+}
+
+/* The reverse sweep's first-term survivors, in full-lane batches: each lane
+ * takes one cell, rebuilds its screen value, drops it if its p is exactly 0
+ * (already in the zero class), and applies the tier-1 bound (first term
+ * minus the screen margin plus a lower bound of the tail, -inf below -708
+ * where sums may be subnormal); cells within Uw move to the tier-1 list,
+ * which is flushed through tier 2 whenever it would overflow. */
+template <int NT>
+__device__ __forceinline__ void flush_first(const DevJob &J, const Lds &S, const char *Hb, uint2 *cand0, uint32_t &n0,
+                                            uint2 *cand, uint32_t &nc, uint4 *c3, uint32_t &n3, BestKey &xb,
+                                            float &Uw, float &U2w, WaveStats &st) {
+  const uint32_t lane = lane_id(), N = J.N;
+  wave_sync();
+#pragma unroll 1
+  for (uint32_t r0 = 0; r0 < n0; r0 += 64) {
+    bool f = false;
+    uint2 cd = make_uint2(0u, 0u);
+    if (r0 + lane < n0) {
+      cd = cand0[r0 + lane];
+      const uint32_t i = cd.x >> 16, j = cd.x & 0xFFFFu, k = cd.y;
+      const uint32_t K = J.K[i], n = S.coln[j];
+      if (k <= min(K, n)) {
+        const float x = (J.A32[i] + S.colb[j]) -
+                        ((*(const float *)(Hb + 4u * k) + *(const float *)(Hb + 4u * (K - k))) +
+                         (*(const float *)(Hb + 4u * (n - k)) + *(const float *)(Hb + 4u * ((N - K) - (n - k)))));
+        if (!(x + J.margin < -745.2f && exact_zero(J, K, n, k, x))) {
+          float L = x - J.margin;
+          if (L >= -708.f) {
+            L += tail_lower_log(N, K, n, k);
+            f = L <= Uw;
+          } else {
+            f = true; /* possibly subnormal: no usable lower bound */
+          }
+        }
+      }
+    }
+    const uint64_t m = __ballot(f);
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    if (cnt) {
+      if (nc + cnt > (uint32_t)kCandCap) {
+        uint64_t tf = cycles();
+        flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
+        st.cyc[3] += cycles() - tf;
+      }
+      if (f) cand[nc + mask_lt(m)] = cd;
+      nc += cnt;
+      st.push1 += cnt;
+    }
+  }
+  n0 = 0;
   wave_sync();
 }
 
@@ -701,19 +769,6 @@ __device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, cons
   __builtin_amdgcn_sched_barrier(0);
 }
 
-/* True when a cell's p is exactly 0.0 in the reference: k > min(K, n) (the
- * tail is empty), or every term from k on is below the exp underflow point
- * (-745.1332...): x + margin < -745.2 bounds the first exponent and a ratio
- * below 1 makes every later term smaller. */
-__device__ __forceinline__ bool exact_zero(const DevJob &J, uint32_t K, uint32_t n, uint32_t k, float x) {
-  const uint32_t N = J.N;
-  if (k > min(K, n)) return true;
-  if (!(x + J.margin < -745.2f)) return false;
-  const uint64_t num = (uint64_t)(K - k) * (uint64_t)(n - k);
-  const uint64_t den = (uint64_t)(k + 1) * (uint64_t)((N - K) - (n - k) + 1);
-  return num < den;
-}
-
 /* The p == 0 cell order (optimize_main.rs:93-116 at equal p): larger k, then
  * smaller (i, j). */
 struct ZeroKey {
@@ -734,8 +789,8 @@ __device__ __forceinline__ ZeroKey wave_zero(ZeroKey z) {
 }
 
 /* Exact best among the p == 1 cells (k <= max(0, K+n-N)): larger k, then
- * smaller (i, j).  Only needed when no cell has p < 1; rebuilds the wave's
- * column counts row by row (they are empty after the reverse sweep). */
+ * smaller (i, j).  Only needed when no cell has p < 1; clears the wave's
+ * column counts and rebuilds them row by row. */
 template <int NT>
 __device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout &Lo, const Lds &S, uint32_t *C) {
   constexpr uint32_t W = NT / 64;
@@ -743,6 +798,8 @@ __device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout 
   const uint32_t T1 = J.T1, T2 = J.T2, T2w = J.T2w, N = J.N;
   const uint32_t c2 = (T2w + 63) / 64, wbeg = lane * c2;
   ZeroKey o = {0u, kNone};
+  for (uint32_t t = lane; t < Lo.T2w_pad; t += 64) C[t] = 0u;
+  wave_sync();
   uint32_t added = 0;
   for (uint32_t i = w; i < T1; i += W) {
     rows_update(S, C, added, i + 1, true);
@@ -917,15 +974,33 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   float U2w = Ustar; /* best tier-2 upper end seen by this wave */
 
   /* ---- E: reverse sweep over this wave's rows that can hold the minimum ---- */
-  static_assert(kCandCap >= 64, "one ballot (64 cells) must fit an empty tier-1 list");
+  static_assert(kCandCap >= 64 && kCand0Cap >= 64, "one ballot (64 cells) must fit an empty list");
+  uint2 *cand0 = S.cand0 + w * kCand0Cap;
   uint2 *cand = S.cand + w * kCandCap;
   uint4 *c3 = S.cand3 + w * kCand3Cap;
-  uint32_t nc = 0, n3 = 0;
-  /* C holds rows [0, added); the last owned row is added - 1 */
+  uint32_t n0 = 0, nc = 0, n3 = 0;
+  /* C holds rows [0, ctop).  The owned rows go from the top down, 64 per
+   * batch: lane l holds the bound of the batch's l-th row, a ballot finds the
+   * next row that can still hold the minimum (Uw only tightens), and the
+   * rows above it are removed in one pass.  Rows below the last rescanned
+   * row are never removed (the next permutation clears the counts). */
+  uint32_t ctop = added;
+  const uint32_t nown = added > w ? (added - 1u - w) / W + 1u : 0u; /* owned rows */
 #pragma unroll 1
-  for (int32_t ii = (int32_t)added - 1; ii >= (int32_t)w; ii -= (int32_t)W) {
-    const uint32_t i = (uint32_t)ii;
-    if (S.rowmin[i] <= Uw) {
+  for (uint32_t b0 = 0; b0 < nown; b0 += 64) {
+    const uint32_t top = added - 1u - b0 * W; /* this batch's first (highest) row */
+    const bool have = b0 + lane < nown;
+    const float rm = have ? S.rowmin[top - lane * W] : __builtin_inff();
+    uint64_t done = 0;
+#pragma unroll 1
+    for (;;) {
+      const uint64_t m = __ballot(rm <= Uw) & ~done;
+      if (!m) break;
+      const uint32_t l = (uint32_t)__builtin_ctzll(m);
+      done |= (2ull << l) - 1ull; /* rows at or above this one are passed */
+      const uint32_t i = top - l * W;
+      rows_update(S, C, i + 1u, ctop, false);
+      ctop = i + 1u;
       st.rows++;
       const uint32_t K = J.K[i];
       const float A = J.A32[i];
@@ -954,43 +1029,32 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
 #pragma unroll
           for (uint32_t c = 0; c < kSegCells; ++c) surv |= (xx[c] <= thr && 2u * t0 + c >= resume) ? 1u << c : 0u;
           if (!__ballot(surv != 0u)) continue;
+          st.cheap += (uint32_t)__popc(surv);
+          uint32_t stot = 0;
+#pragma unroll
+          for (uint32_t c = 0; c < kSegCells; ++c) stot += (uint32_t)__popcll(__ballot(((surv >> c) & 1u) != 0u));
+          /* make room first; only a segment with more survivors than the list
+           * holds goes through the flush-and-resume path below */
+          if (n0 + stot > (uint32_t)kCand0Cap) flush_first<NT>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st);
 #pragma unroll
           for (uint32_t c = 0; c < kSegCells; ++c) {
-            bool f = false;
-            if (surv & (1u << c)) {
-              /* p == 0 cells are already in the zero class */
-              if (!(xx[c] < uthr && exact_zero(J, K, nn4[c] >> 2, kk[c], xx[c]))) {
-                float L = xx[c] - margin;
-                if (L >= -708.f) {
-                  L += tail_lower_log(J.N, K, nn4[c] >> 2, kk[c]);
-                  f = L <= Uw;
-                } else {
-                  f = true; /* possibly subnormal: no usable lower bound */
-                }
-              }
-            }
-            f = f && !again;
+            const bool f = ((surv >> c) & 1u) != 0u && !again;
             const uint64_t m = __ballot(f);
             const uint32_t cnt = (uint32_t)__popcll(m);
-            if (nc + cnt > (uint32_t)kCandCap) {
+            if (n0 + cnt > (uint32_t)kCand0Cap) {
               again = true;
               resume = 2u * t0 + c;
             } else if (cnt) {
-              if (f) cand[nc + mask_lt(m)] = make_uint2((i << 16) | (2u * (wbeg + t0) + c), kk[c]);
-              nc += cnt;
-              st.push1 += cnt;
+              if (f) cand0[n0 + mask_lt(m)] = make_uint2((i << 16) | (2u * (wbeg + t0) + c), kk[c]);
+              n0 += cnt;
             }
           }
         }
-        if (again) {
-          uint64_t tf = cycles();
-          flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
-          st.cyc[3] += cycles() - tf;
-        }
+        /* again: the next pass starts by flushing the full list */
       }
     }
-    rows_update(S, C, i >= W ? i - W + 1 : 0u, i + 1, false);
   }
+  if (n0) flush_first<NT>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st);
   uint64_t tc3 = cycles();
   st.cyc[2] += tc3 - tc2;
   if (nc) flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
@@ -1030,8 +1094,11 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
 
 /* Adds one wave's work counters to the job's counters (one atomic each per wave). */
 __device__ __forceinline__ void flush_stats(const DevJob &J, const WaveStats &st) {
-  uint32_t t = st.terms;
-  for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off);
+  uint32_t t = st.terms, ch = st.cheap;
+  for (int off = 32; off >= 1; off >>= 1) {
+    t += __shfl_xor(t, off);
+    ch += __shfl_xor(ch, off);
+  }
   if (lane_id() == 0 && J.stats) {
     if (st.cells) atomicAdd(&J.stats[0], (unsigned long long)st.cells);
     if (t) atomicAdd(&J.stats[1], (unsigned long long)t);
@@ -1042,6 +1109,7 @@ __device__ __forceinline__ void flush_stats(const DevJob &J, const WaveStats &st
     if (st.push2) atomicAdd(&J.stats[9], (unsigned long long)st.push2);
     if (st.flushes) atomicAdd(&J.stats[10], (unsigned long long)st.flushes);
     if (st.gap) atomicAdd(&J.stats[11], (unsigned long long)st.gap);
+    if (ch) atomicAdd(&J.stats[12], (unsigned long long)ch);
   }
 }
 
@@ -1087,7 +1155,7 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
   load_job_tables<NT, HLDS>(J, S);
   __syncthreads();
   uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * J.N2;
-  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u};
+  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
   for (uint64_t pp = blockIdx.x; pp < n_perm; pp += gridDim.x) {
     const uint64_t pid = perm_base + pp;
     const uint32_t *i1 = (EXPLICIT && idx1) ? idx1 + pp * J.N1 : nullptr;
@@ -1144,7 +1212,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs,
   __shared__ uint32_t s_item;
   uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * ws_stride;
   uint32_t cur = kNone;
-  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u};
+  WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
   const DevJob *last = nullptr;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
@@ -1160,7 +1228,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs,
     const Lds S = lds_view(smem, B.lo, W);
     if (wi.job != cur) {
       if (last) flush_stats(*last, st);
-      st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u};
+      st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
       last = &B.J;
       load_job_tables<NT, HLDS>(B.J, S);
       __syncthreads();
