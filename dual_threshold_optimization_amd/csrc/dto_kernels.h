@@ -46,6 +46,8 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+/* The wave's index in the workgroup, as a wave-uniform (scalar) value. */
+__device__ __forceinline__ uint32_t wave_id() { return (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
 __device__ __forceinline__ uint32_t mask_lt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -110,7 +112,7 @@ __device__ __forceinline__ float wave_minf(float v) {
 /* Exclusive scan of data[0..len) in LDS, total written to data[len]. */
 template <int NT>
 __device__ __forceinline__ void block_excl_scan(uint32_t *data, uint32_t len, uint32_t *wave_sums) {
-  const uint32_t tid = threadIdx.x, w = tid >> 6;
+  const uint32_t tid = threadIdx.x, w = wave_id();
   const uint32_t per = (len + NT - 1) / NT;
   const uint32_t beg = min(tid * per, len), end = min(beg + per, len);
   uint32_t s = 0;
@@ -208,7 +210,7 @@ __device__ __forceinline__ void load_job_tables(const DevJob &J, const Lds &S) {
 template <int NT, typename Fn>
 __device__ __forceinline__ void shuffle_visit(const Shuffle &s, uint32_t *wsum, Fn fn) {
   constexpr uint32_t kU = 8, W = NT / 64;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   uint32_t carried = 0;
   for (uint32_t base = 0; base < s.dom; base += NT * kU) {
     uint32_t v[kU], cnt = 0;
@@ -248,7 +250,7 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
                                            const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
                                            uint32_t *gpairs) {
   constexpr int W = NT / 64;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   const uint32_t T1 = J.T1, T2 = J.T2, T2wp = Lo.T2w_pad;
   uint32_t *lpairs = S.pairs;
   volatile uint32_t *vpairs = gpairs; /* global staging: bypass the per-CU cache on the way back */
@@ -360,17 +362,19 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
 }
 
 /* Adds or removes the pairs of rows [ia, ib) (contiguous in S.nb) into a
- * wave's column counts (four bins in flight per lane). */
+ * wave's column counts (four bins in flight per lane).  No wait at the end:
+ * the wave's later LDS reads of C are processed after these updates (the
+ * LDS executes one wave's operations in order). */
 __device__ __forceinline__ void rows_update(const Lds &S, uint32_t *C, uint32_t ia, uint32_t ib, bool add) {
   const uint32_t lane = lane_id();
   const uint32_t e1 = S.rowstart[ib];
   for (uint32_t e = S.rowstart[ia] + lane; e < e1; e += 256) {
     uint32_t b[4];
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) b[u] = e + 64u * u < e1 ? (uint32_t)S.nb[e + 64u * u] : 0xFFFFu;
+    for (uint32_t u = 0; u < 4; ++u) b[u] = S.nb[min(e + 64u * u, e1 - 1u)];
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) {
-      if (b[u] != 0xFFFFu) {
+      if (e + 64u * u < e1) {
         const uint32_t inc = 1u << ((b[u] & 1u) * 16u);
         if (add)
           atomicAdd(&C[b[u] >> 1], inc);
@@ -379,7 +383,6 @@ __device__ __forceinline__ void rows_update(const Lds &S, uint32_t *C, uint32_t 
       }
     }
   }
-  wave_sync();
 }
 
 /* Adds (sign +1) or removes (-1) row i's pairs into a wave's column counts. */
@@ -438,9 +441,10 @@ __device__ __forceinline__ float tail_lower_log(uint32_t N, uint32_t K, uint32_t
 }
 
 /* Bracket of S = sum_{i >= k} t_i / t_k from the ratio recurrence: up to mmax
- * terms in FP32, then a geometric bound of the rest (the ratio only falls
- * with i).  Accumulated relative error of the products and the sum is at most
- * 9 (m+1) 2^-24; hi is +inf when the terms are still growing at the cut. */
+ * terms in FP32 (fewer once the rest is below 2^-16 S), then a geometric
+ * bound of the rest (the ratio only falls with i).  Accumulated relative
+ * error of the products and the sum is at most 9 (m+1) 2^-24; hi is +inf
+ * when the terms are still growing at the cut. */
 struct TailBracket {
   float lo, hi;
 };
@@ -455,7 +459,9 @@ __device__ __forceinline__ TailBracket tail_bracket(uint32_t N, uint32_t K, uint
     S += P;
     ++i;
     ++m;
-    if ((P <= S * 0x1p-30f && r <= 0.9f) || P > 1e30f) break;
+    if (P > 1e30f) break;
+    /* the geometric bound of the rest (ratios only fall) is below 2^-16 S */
+    if (r < 0.999f && P * r <= S * (1.f - r) * 0x1p-16f) break;
   }
   const float eps = (float)(9u * (m + 1u)) * 0x1p-24f + 1e-7f;
   float rem = 0.f;
@@ -794,7 +800,7 @@ __device__ __forceinline__ ZeroKey wave_zero(ZeroKey z) {
 template <int NT>
 __device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout &Lo, const Lds &S, uint32_t *C) {
   constexpr uint32_t W = NT / 64;
-  const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t w = wave_id(), lane = lane_id();
   const uint32_t T1 = J.T1, T2 = J.T2, T2w = J.T2w, N = J.N;
   const uint32_t c2 = (T2w + 63) / 64, wbeg = lane * c2;
   ZeroKey o = {0u, kNone};
@@ -831,7 +837,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
                                  const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
                                  WaveStats &st, uint32_t *gpairs) {
   constexpr uint32_t W = NT / 64;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   uint64_t tc0 = cycles();
   build_rows<NT, EXPLICIT, HLDS, PL>(J, Lo, S, i1, i2, s1, s2, err, gpairs);
   uint64_t tc1 = cycles();
@@ -1260,7 +1266,7 @@ __global__ __launch_bounds__(NT) void k_grid_counts(DevJob J, LdsLayout Lo, cons
   const Lds S = lds_view(smem, Lo, W);
   Shuffle s1, s2;
   build_rows<NT, true, false, PL>(J, Lo, S, idx1, idx2, s1, s2, err, ws);
-  const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t w = wave_id(), lane = lane_id();
   const uint32_t T1 = J.T1, T2 = J.T2, T2w = J.T2w;
   uint32_t *C = S.cw + w * Lo.T2w_pad;
   const uint32_t c2 = (T2w + 63) / 64, wbeg = lane * c2;
