@@ -6,6 +6,7 @@ import sys
 
 from . import _lib
 from .dto import Job
+from .read import read_ranked_feature_list_from_csv
 from .stat_operations import result_json
 
 
@@ -31,10 +32,20 @@ def main(argv=None):
     print("Permutations: %d" % a.permutations, file=err)
     print("Threads: %d" % threads, file=err)
     print("Multi-node mode: %s" % ("enabled" if a.multi_node else "disabled"), file=err)
+    product = ("The product of the lengths of the threshold lists (this describes the asymptotic runtime of a "
+               "single job): %d")
     try:
-        job = Job.from_files(a.ranked_list1, a.ranked_list2, a.background, a.drop_outside_background)
-        print("The product of the lengths of the threshold lists (this describes the asymptotic runtime of a "
-              "single job): %d" % (job.n_thresholds1 * job.n_thresholds2), file=err)
+        try:
+            job = Job.from_files(a.ranked_list1, a.ranked_list2, a.background, a.drop_outside_background)
+        except _lib.DtoError as e:
+            # main.rs prints the product after reading both lists, before the
+            # background and population checks
+            if e.code not in (_lib.DTO_ERR_IO, _lib.DTO_ERR_PARSE):
+                l1 = read_ranked_feature_list_from_csv(a.ranked_list1)
+                l2 = read_ranked_feature_list_from_csv(a.ranked_list2)
+                print(product % (len(l1.thresholds()) * len(l2.thresholds())), file=err)
+            raise
+        print(product % (job.n_thresholds1 * job.n_thresholds2), file=err)
         if a.multi_node:
             raise _lib.DtoError(1, "Multi-node mode requires MPI. Rebuild with the 'mpi' feature enabled.")
         seed = a.seed if a.seed is not None else int.from_bytes(os.urandom(8), "little")

@@ -511,7 +511,7 @@ int dto_job_info_get(const dto_job *job, dto_job_info *info) {
 }
 
 int dto_job_thresholds(const dto_job *job, int list, uint32_t *out, uint32_t cap) {
-  if (!job || (list != 1 && list != 2)) return fail(DTO_ERR_INVALID_ARGUMENT, "bad job or list");
+  if (!job || (list != 1 && list != 2)) return -fail(DTO_ERR_INVALID_ARGUMENT, "bad job or list");
   const std::vector<uint32_t> &t = list == 1 ? job->h.thr1 : job->h.thr2;
   if (out)
     for (uint32_t i = 0; i < cap && i < t.size(); ++i) out[i] = t[i];

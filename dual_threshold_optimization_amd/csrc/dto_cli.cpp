@@ -58,6 +58,37 @@ static bool parse_u64(const std::string &s, uint64_t *v) {
   return true;
 }
 
+/* Largest rank of a CSV the library has already parsed: last field of each
+ * line, the same `usize` parse and `as u32` truncation. */
+static uint32_t max_rank(const std::string &path) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return 0;
+  uint32_t mx = 0;
+  std::string line;
+  int ch;
+  auto take = [&]() {
+    size_t comma = line.find(',');
+    if (comma != std::string::npos) {
+      std::string r = line.substr(comma + 1);
+      size_t a = 0, b = r.size();
+      while (a < b && strchr(" \t\r\n\v\f", r[a])) ++a;
+      while (b > a && strchr(" \t\r\n\v\f", r[b - 1])) --b;
+      uint64_t v = 0;
+      if (parse_u64(r.substr(a, b - a), &v) && (uint32_t)v > mx) mx = (uint32_t)v;
+    }
+    line.clear();
+  };
+  while ((ch = fgetc(f)) != EOF) {
+    if (ch == '\n')
+      take();
+    else
+      line.push_back((char)ch);
+  }
+  if (!line.empty()) take();
+  fclose(f);
+  return mx;
+}
+
 int main(int argc, char **argv) {
   std::string l1, l2, bg;
   bool have_bg = false, multi = false, drop = false, have_seed = false;
@@ -147,13 +178,27 @@ int main(int argc, char **argv) {
   dto_job *job = nullptr;
   int rc = dto_job_create_from_files(l1.c_str(), l2.c_str(), have_bg ? bg.c_str() : nullptr,
                                      drop ? DTO_DROP_OUTSIDE_BACKGROUND : 0u, &job);
+  /* main.rs prints the threshold product after reading both lists and
+   * before the background and population checks: errors past the readers
+   * come after that line */
+  const bool lists_read = rc == DTO_OK || (rc != DTO_ERR_IO && rc != DTO_ERR_PARSE);
+  if (lists_read) {
+    uint64_t t1 = 0, t2 = 0;
+    if (rc == DTO_OK) {
+      dto_job_info info;
+      dto_job_info_get(job, &info);
+      t1 = info.n_thresholds1;
+      t2 = info.n_thresholds2;
+    } else {
+      t1 = dto_generate_thresholds(max_rank(l1), nullptr, 0);
+      t2 = dto_generate_thresholds(max_rank(l2), nullptr, 0);
+    }
+    fprintf(stderr,
+            "The product of the lengths of the threshold lists (this describes the asymptotic runtime of a single "
+            "job): %llu\n",
+            (unsigned long long)(t1 * t2));
+  }
   if (rc) panic(dto_last_error());
-  dto_job_info info;
-  dto_job_info_get(job, &info);
-  fprintf(stderr,
-          "The product of the lengths of the threshold lists (this describes the asymptotic runtime of a single "
-          "job): %llu\n",
-          (unsigned long long)info.n_thresholds1 * info.n_thresholds2);
   if (multi) panic("Multi-node mode requires MPI. Rebuild with the 'mpi' feature enabled.");
   if (!have_seed) seed = (uint64_t)time(nullptr) ^ ((uint64_t)clock() << 32);
   dto_result res;

@@ -131,7 +131,9 @@ int dto_job_create_from_files(const char *path1, const char *path2, const char *
 int dto_job_set_thresholds(dto_job *job, const uint32_t *thr1, uint32_t t1, const uint32_t *thr2,
                            uint32_t t2);
 int dto_job_info_get(const dto_job *job, dto_job_info *info);
-/* Threshold values (RankedFeatureList::thresholds, ranked.rs:261-263). */
+/* Threshold values of list 1 or 2 (RankedFeatureList::thresholds,
+ * ranked.rs:261-263): writes up to cap values into out (may be NULL) and
+ * returns how many the list has, or minus a status code on error. */
 int dto_job_thresholds(const dto_job *job, int list, uint32_t *out, uint32_t cap);
 void dto_job_destroy(dto_job *job);
 

@@ -3,6 +3,15 @@ import sys
 
 import pytest
 
+# PyTorch ships its own HIP runtime.  Imported first, it is the runtime the
+# library binds to (same soname), so torch tensors, streams and events and
+# the library share one runtime; a process that loads the library first and
+# torch later ends up with two runtimes, and torch then finds no GPU.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
