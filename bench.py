@@ -5,7 +5,8 @@ Headline (the metric's configuration): the genome-scale job -- lists of
 drop), 600 planted shared features, 10% ties -- run as the reference CLI
 runs it: the unpermuted task plus 10,000 permutations, here with the
 10,000 explicit permutations already resident in HBM.  One step = one whole
-job (both launches); value = permutations per second.
+job: one launch of the 10,001 tasks (task 0 is the identity permutation);
+value = permutations per second.
 
 Extra lines on the same JSON object: the plain FP64 kernel (every grid
 cell exact), on-device Philox shuffles, the 512-job batched persistent
@@ -104,12 +105,14 @@ def main():
                                                            job.n_thresholds2, time.perf_counter() - t))
     gen = torch.Generator(device=dev)
     gen.manual_seed(2)
-    idx1 = torch.argsort(torch.rand(P, N1, device=dev, generator=gen), dim=1).to(torch.int32).contiguous()
-    idx2 = torch.argsort(torch.rand(P, N2, device=dev, generator=gen), dim=1).to(torch.int32).contiguous()
-    ident1 = torch.arange(N1, dtype=torch.int32, device=dev)
-    ident2 = torch.arange(N2, dtype=torch.int32, device=dev)
-    out = torch.empty(P * RECORD_BYTES, dtype=torch.uint8, device=dev)
-    out0 = torch.empty(RECORD_BYTES, dtype=torch.uint8, device=dev)
+    # row 0: the identity (task 0, the unpermuted run); rows 1..P: the permutations.
+    # One launch runs all P + 1 tasks, as run_single_node runs them on one pool.
+    idx1 = torch.argsort(torch.rand(P + 1, N1, device=dev, generator=gen), dim=1).to(torch.int32)
+    idx2 = torch.argsort(torch.rand(P + 1, N2, device=dev, generator=gen), dim=1).to(torch.int32)
+    idx1[0] = torch.arange(N1, dtype=torch.int32, device=dev)
+    idx2[0] = torch.arange(N2, dtype=torch.int32, device=dev)
+    idx1, idx2 = idx1.contiguous(), idx2.contiguous()
+    out_all = torch.empty((P + 1) * RECORD_BYTES, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
     torch.cuda.synchronize()
@@ -118,10 +121,9 @@ def main():
         job.run_device(n, o.data_ptr(), d_idx1=i1.data_ptr(), d_idx2=i2.data_ptr(), mode=mode, stream=sptr)
 
     def step(mode="screened", events=None):
-        launch(mode, 1, ident1, ident2, out0)  # task 0: unpermuted
         if events is not None:
             events[0].record(stream)
-        launch(mode, P, idx1, idx2, out)
+        launch(mode, P + 1, idx1, idx2, out_all)
         if events is not None:
             events[1].record(stream)
 
@@ -138,13 +140,13 @@ def main():
     wall = t1 - t0
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
     value = P * args.steps / wall
-    rec = np.frombuffer(out.cpu().numpy().tobytes(), dtype=dto._lib.RECORD_DTYPE)
-    unperm = np.frombuffer(out0.cpu().numpy().tobytes(), dtype=dto._lib.RECORD_DTYPE)[0]
+    allrec = np.frombuffer(out_all.cpu().numpy().tobytes(), dtype=dto._lib.RECORD_DTYPE)
+    unperm, rec = allrec[0], allrec[1:]
     emp = float(np.mean(rec["pvalue"] <= unperm["pvalue"]))
     log("screened explicit: %.1f ms/step, kernel %.2f ms, %.0f perms/s, unpermuted p=%r empirical=%r"
         % (wall / args.steps * 1e3, kern_ms, value, float(unperm["pvalue"]), emp))
 
-    bytes_per_launch = P * (4 * N1 + 4 * N2 + RECORD_BYTES)
+    bytes_per_launch = (P + 1) * (4 * N1 + 4 * N2 + RECORD_BYTES)
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": read_traffic(args.traffic_file),
@@ -168,10 +170,12 @@ def main():
 
     # ---- same job: plain FP64 kernel (every cell exact) and Philox shuffles ----
     Pp = min(args.plain_perms, P)
-    launch("plain", Pp, idx1, idx2, out)
+    i1p, i2p = idx1[1:], idx2[1:]
+    out = torch.empty(P * RECORD_BYTES, dtype=torch.uint8, device=dev)
+    launch("plain", Pp, i1p, i2p, out)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    launch("plain", Pp, idx1, idx2, out)
+    launch("plain", Pp, i1p, i2p, out)
     torch.cuda.synchronize()
     plain_rate = Pp / (time.perf_counter() - t)
     rec_plain = np.frombuffer(out[:Pp * RECORD_BYTES].cpu().numpy().tobytes(), dtype=dto._lib.RECORD_DTYPE)
@@ -190,7 +194,7 @@ def main():
                             "screened_philox_perms_per_s": round(philox_rate, 2),
                             "screened_vs_plain": round(value / plain_rate, 2),
                             "screened_equals_plain_on_%d_perms" % Pp: same}
-    del idx1, idx2
+    del idx1, idx2, i1p, i2p, out_all
     torch.cuda.empty_cache()
 
     # ---- dense thresholds: 2,000 x 2,000 custom thresholds on the same job (4M cells) ----

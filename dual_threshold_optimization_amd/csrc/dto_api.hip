@@ -129,6 +129,7 @@ int upload_job(dto_job *j) {
   size_t o_t1 = put(buf, h.thr1), o_t2 = put(buf, h.thr2), o_LF = put(buf, h.LF), o_D = put(buf, h.D);
   size_t o_H = put(buf, h.H32), o_A = put(buf, h.A32), o_B = put(buf, h.B32);
   size_t o_err = put(buf, std::vector<int>(4, 0));
+  size_t o_ctr = put(buf, std::vector<uint32_t>(kCounterRing, 0));
   size_t o_stats = put(buf, std::vector<unsigned long long>(16, 0));
   if (j->arena) {
     (void)hipFree(j->arena);
@@ -163,6 +164,7 @@ int upload_job(dto_job *j) {
   d.A32 = (const float *)(b + o_A);
   d.B32 = (const float *)(b + o_B);
   j->d_err = (int *)(b + o_err);
+  j->d_ctr = (uint32_t *)(b + o_ctr);
   d.stats = (unsigned long long *)(b + o_stats);
   /* LDS: keep the screen table resident when it fits */
   const size_t lim = pr.lds_per_block;
@@ -193,8 +195,11 @@ static hipError_t launch_opt(const dto_job *j, const LdsLayout &lo, uint32_t gri
                              uint32_t job_id, int32_t permuted, dto_record *out) {
   auto kern = k_optimize<kNT, E, HL, PL, PA, RC>;
   (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lo.total);
+  /* a slot of the job's counter ring for this launch, zeroed on its stream */
+  uint32_t *ctr = j->d_ctr + (const_cast<dto_job *>(j)->ctr_next.fetch_add(1u) % kCounterRing);
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, ctr);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kNT), lo.total, st, j->dev, lo, i1, i2, n_perm, base, seed, job_id,
-                     permuted, out, j->d_err, j->ws);
+                     permuted, out, j->d_err, j->ws, ctr);
   return hipGetLastError();
 }
 
@@ -234,6 +239,7 @@ static int launch_perms(dto_job *j, uint32_t mode, bool expl, const uint32_t *d1
                         uint64_t base, uint64_t seed, uint32_t job_id, int32_t permuted, dto_record *d_out,
                         hipStream_t st) {
   if (n_perm == 0) return DTO_OK;
+  if (n_perm >= (1ull << 32) - 65536ull) return fail(DTO_ERR_TOO_LARGE, "too many permutations in one launch");
   hipError_t e;
   if (mode == DTO_MODE_PLAIN) {
     const LdsLayout &lo = j->lo_plain;

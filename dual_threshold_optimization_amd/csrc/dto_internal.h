@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 
 #include "dto_device.h"
@@ -28,6 +29,8 @@ int device_props(int dev, DevProps *out);
 LdsLayout make_layout(const HostJob &h, int W, bool with_h, size_t lds_limit);
 }  // namespace dto
 
+constexpr uint32_t kCounterRing = 256;
+
 /* A job: host tables plus their device copy (one HBM arena). */
 struct dto_job {
   dto::HostJob h;
@@ -35,6 +38,11 @@ struct dto_job {
   void *arena = nullptr;
   size_t arena_bytes = 0;
   int *d_err = nullptr;
+  /* work counters of permutation launches: a ring of kCounterRing slots, one
+   * per launch, so that launches on different streams do not share one (a
+   * job with more launches than that in flight at once would) */
+  uint32_t *d_ctr = nullptr;
+  std::atomic<uint32_t> ctr_next{0};
   int device = -1;
   dto::LdsLayout lo_screen{}, lo_plain{};
   uint32_t *ws = nullptr; /* global pair staging (only when the LDS cannot hold the pairs) */

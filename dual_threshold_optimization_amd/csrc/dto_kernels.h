@@ -1148,21 +1148,31 @@ __device__ __forceinline__ void write_record(const DevJob &J, const BestKey &b, 
 /* Permutations [perm_base, perm_base + n_perm) of one job.  EXPLICIT: idx1 /
  * idx2 hold n_perm permutations back to back (NULL: identity); otherwise the
  * Philox shuffle keyed by (seed, job_id, permutation id, list).  ws: per
- * workgroup pair staging (N2 words each) when !PL. */
+ * workgroup pair staging (N2 words each) when !PL.  Workgroups take
+ * permutations from a device counter (zeroed before the launch), so that
+ * permutations of uneven cost spread evenly; every workgroup exits once the
+ * counter passes n_perm. */
 template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS>
 __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const uint32_t *__restrict__ idx1,
                                                  const uint32_t *__restrict__ idx2, uint64_t n_perm,
                                                  uint64_t perm_base, uint64_t seed, uint32_t job_id,
                                                  int32_t permuted, dto_record *__restrict__ out,
-                                                 int *__restrict__ err, uint32_t *__restrict__ ws) {
+                                                 int *__restrict__ err, uint32_t *__restrict__ ws,
+                                                 uint32_t *__restrict__ counter) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t s_next;
   constexpr int W = NT / 64;
   const Lds S = lds_view(smem, Lo, W);
   load_job_tables<NT, HLDS>(J, S);
   __syncthreads();
   uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * J.N2;
   WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
-  for (uint64_t pp = blockIdx.x; pp < n_perm; pp += gridDim.x) {
+  for (;;) {
+    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint64_t pp = s_next;
+    __syncthreads();
+    if (pp >= n_perm) break;
     const uint64_t pid = perm_base + pp;
     const uint32_t *i1 = (EXPLICIT && idx1) ? idx1 + pp * J.N1 : nullptr;
     const uint32_t *i2 = (EXPLICIT && idx2) ? idx2 + pp * J.N2 : nullptr;
@@ -1175,6 +1185,12 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
     if (threadIdx.x == 0) write_record(J, b, permuted, out + pp, err);
   }
   flush_stats(J, st);
+}
+
+/* Zeroes one work counter (a kernel, not a memset, so that callers may
+ * capture the launch sequence in a graph). */
+static __global__ void k_zero_counter(uint32_t *counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
 }
 
 /* ------------------------------------------------------------------ */
