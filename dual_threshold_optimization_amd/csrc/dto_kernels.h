@@ -258,7 +258,7 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
   for (uint32_t t = tid; t < W * T2wp; t += NT) S.cw[t] = 0;
   /* A: list1 feature m sits at the position k with idx1[k] == m (loads of
    * kU positions in flight per thread) */
-  constexpr uint32_t kU = 8;
+  constexpr uint32_t kU = 16;
   if (!EXPLICIT) {
     shuffle_visit<NT>(s1, S.redS, [&](uint32_t k, uint32_t m) { S.nb[m] = (uint16_t)J.bin1[k]; });
   } else {
