@@ -86,6 +86,7 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
   region(L.cand3, 16u * (uint32_t)W * (uint32_t)kCand3Cap);
   region(L.exptab, 8u * 256u);
   region(L.red, 80u * (uint32_t)W);
+  region(L.misc, 16u);
   uint32_t base = 0;
   L.h = L.pairs = 0;
   if (with_h) {
@@ -101,10 +102,13 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
     L.pairs_in_lds = 0;
   }
   for (uint32_t *f : {&L.nb, &L.rowstart, &L.cursor, &L.cw, &L.rowmin, &L.coln, &L.colb, &L.cand, &L.cand3,
-                      &L.exptab, &L.red})
+                      &L.exptab, &L.red, &L.misc})
     *f += base;
   L.cand0 = L.cursor;
   L.total = base + fixed;
+  /* the batch kernel keeps its work item in the largest layout's misc slot,
+   * which must therefore end the layout */
+  if (L.misc + 16u != L.total) abort();
   return L;
 }
 

@@ -41,7 +41,9 @@ hipError_t launch_batch_t(uint32_t grid, uint32_t lds, hipStream_t st, const Bat
                           uint32_t ws_stride) {
   auto kern = k_batch<NT, HL, PL, RC>;
   (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride);
+  /* every layout ends with its 16-byte misc slot: the largest one's is free in all */
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride,
+                     lds - 16u);
   return hipGetLastError();
 }
 

@@ -40,8 +40,8 @@ struct DevJob {
 
 /* Byte offsets of the dynamic LDS regions of one workgroup. */
 struct LdsLayout {
-  uint32_t h;        /* f32[N+1] screen table (only when h_in_lds) */
-  uint32_t pairs;    /* u32[N2] (a,b) staging: aliases h when h_in_lds */
+  uint32_t h;        /* f32[N+1] screen table (only when h_in_lds); always 0 */
+  uint32_t pairs;    /* u32[N2] (a,b) staging: aliases h; always 0 */
   uint32_t pairs_in_lds;
   uint32_t nb;       /* u16[max(N1,N2)]: list1 bins after the shuffle, then row-sorted list2 bins */
   uint32_t rowstart; /* u32[T1+1] */
@@ -55,6 +55,7 @@ struct LdsLayout {
   uint32_t cand3;    /* uint4[W][kCand3Cap] tier-2 survivors */
   uint32_t exptab;   /* u64[256] */
   uint32_t red;      /* per-wave reduction slots */
+  uint32_t misc;     /* u32[4]: the workgroup's current work item */
   uint32_t total;
   uint32_t T2w_pad;
   uint32_t h_in_lds;

@@ -109,6 +109,28 @@ __device__ __forceinline__ float wave_minf(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(x, 63));
 }
 
+/* Minimum over the wave of each of R values, returned in every lane; the R
+ * reductions are interleaved so that their DPP latencies overlap. */
+template <int R>
+__device__ __forceinline__ void wave_minf_n(float *v) {
+  const int inf = __float_as_int(__builtin_inff());
+  int x[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) x[r] = __float_as_int(v[r]);
+#define DTO_MIN_STEP(ctrl, rmask)                                                                                  \
+  _Pragma("unroll") for (int r = 0; r < R; ++r) x[r] = __float_as_int(                                            \
+      fminf(__int_as_float(x[r]), __int_as_float(__builtin_amdgcn_update_dpp(inf, x[r], ctrl, rmask, 0xf, false))));
+  DTO_MIN_STEP(0x111, 0xf)
+  DTO_MIN_STEP(0x112, 0xf)
+  DTO_MIN_STEP(0x114, 0xf)
+  DTO_MIN_STEP(0x118, 0xf)
+  DTO_MIN_STEP(0x142, 0xa)
+  DTO_MIN_STEP(0x143, 0xc)
+#undef DTO_MIN_STEP
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = __int_as_float(__builtin_amdgcn_readlane(x[r], 63));
+}
+
 /* Exclusive scan of data[0..len) in LDS, total written to data[len]. */
 template <int NT>
 __device__ __forceinline__ void block_excl_scan(uint32_t *data, uint32_t len, uint32_t *wave_sums) {
@@ -151,8 +173,10 @@ struct Lds {
 
 __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo, int W) {
   Lds s;
-  s.h = (float *)(smem + Lo.h);
-  s.pairs = (uint32_t *)(smem + Lo.pairs);
+  /* the screen table and the pair staging start the dynamic region (their
+   * layout offsets are always 0), so table addresses are the scaled indices */
+  s.h = (float *)smem;
+  s.pairs = (uint32_t *)smem;
   s.nb = (uint16_t *)(smem + Lo.nb);
   s.rowstart = (uint32_t *)(smem + Lo.rowstart);
   s.cursor = (uint32_t *)(smem + Lo.cursor);
@@ -169,6 +193,16 @@ __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo
   s.redS = (uint32_t *)(smem + Lo.red + 16 * W + 3 * W * sizeof(BestKey));
   s.redU2 = (float *)(smem + Lo.red + 16 * W + 3 * W * sizeof(BestKey) + 4 * W);
   return s;
+}
+
+/* One screen table entry by byte offset.  In LDS the table starts the
+ * dynamic region, which is LDS address 0 in every kernel that reads it (none
+ * of them declares static LDS), so the offset is the LDS address itself and
+ * needs no base register or add. */
+template <bool HLDS>
+__device__ __forceinline__ float htab(const char *Hb, uint32_t off) {
+  if (HLDS) return *(const __attribute__((address_space(3))) float *)off;
+  return *(const float *)(Hb + off);
 }
 
 /* Copies n floats global -> LDS with 16-byte loads (both 16-byte aligned). */
@@ -382,6 +416,30 @@ __device__ __forceinline__ void rows_update(const Lds &S, uint32_t *C, uint32_t 
           atomicSub(&C[b[u] >> 1], inc);
       }
     }
+  }
+}
+
+/* Software-pipelined form of rows_update for the forward sweep: the bins of
+ * the first kPreBins * 64 pairs of the next update are read into registers
+ * while the current row is scanned, so that the update itself issues its
+ * adds without waiting for a load. */
+constexpr uint32_t kPreBins = 5;
+
+__device__ __forceinline__ void prefetch_bins(const Lds &S, uint32_t e0, uint32_t e1, uint32_t *pb) {
+  const uint32_t lane = lane_id(), last = e1 > e0 ? e1 - 1u : 0u;
+#pragma unroll
+  for (uint32_t u = 0; u < kPreBins; ++u) pb[u] = S.nb[min(e0 + lane + 64u * u, last)];
+}
+
+__device__ __forceinline__ void add_bins(const Lds &S, uint32_t *C, uint32_t e0, uint32_t e1, const uint32_t *pb) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (uint32_t u = 0; u < kPreBins; ++u)
+    if (e0 + lane + 64u * u < e1) atomicAdd(&C[pb[u] >> 1], 1u << ((pb[u] & 1u) * 16u));
+  /* pairs past the prefetched ones (rows with many pairs) */
+  for (uint32_t e = e0 + 64u * kPreBins + lane; e < e1; e += 64u) {
+    const uint32_t b = S.nb[e];
+    atomicAdd(&C[b >> 1], 1u << ((b & 1u) * 16u));
   }
 }
 
@@ -622,7 +680,7 @@ __device__ __forceinline__ bool exact_zero(const DevJob &J, uint32_t K, uint32_t
  * minus the screen margin plus a lower bound of the tail, -inf below -708
  * where sums may be subnormal); cells within Uw move to the tier-1 list,
  * which is flushed through tier 2 whenever it would overflow. */
-template <int NT>
+template <int NT, bool HLDS>
 __device__ __forceinline__ void flush_first(const DevJob &J, const Lds &S, const char *Hb, uint2 *cand0, uint32_t &n0,
                                             uint2 *cand, uint32_t &nc, uint4 *c3, uint32_t &n3, BestKey &xb,
                                             float &Uw, float &U2w, WaveStats &st) {
@@ -638,8 +696,8 @@ __device__ __forceinline__ void flush_first(const DevJob &J, const Lds &S, const
       const uint32_t K = J.K[i], n = S.coln[j];
       if (k <= min(K, n)) {
         const float x = (J.A32[i] + S.colb[j]) -
-                        ((*(const float *)(Hb + 4u * k) + *(const float *)(Hb + 4u * (K - k))) +
-                         (*(const float *)(Hb + 4u * (n - k)) + *(const float *)(Hb + 4u * ((N - K) - (n - k)))));
+                        ((htab<HLDS>(Hb, 4u * k) + htab<HLDS>(Hb, 4u * (K - k))) +
+                         (htab<HLDS>(Hb, 4u * (n - k)) + htab<HLDS>(Hb, 4u * ((N - K) - (n - k)))));
         if (!(x + J.margin < -745.2f && exact_zero(J, K, n, k, x))) {
           float L = x - J.margin;
           if (L >= -708.f) {
@@ -728,7 +786,7 @@ __device__ __forceinline__ uint32_t read_segment(const uint32_t *C, uint32_t w0,
  * below the underflow bound flags every cell that may have p == 0.  Hb is
  * the byte base of the H table; indices are byte offsets (4x), and the two
  * that can wrap for cells of the other classes saturate at 0. */
-template <bool REGCOLS>
+template <bool HLDS, bool REGCOLS>
 __device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, const char *Hb, const uint32_t *wv,
                                              uint32_t nseg, uint32_t w0, uint32_t run, uint32_t K, float A,
                                              const ColRegs &cr, uint32_t *kk, uint32_t *nn4, float *xx) {
@@ -757,10 +815,10 @@ __device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, cons
     kk[c] = k;
     nn4[c] = n4;
     d2[c] = i2;
-    h0[c] = *(const float *)(Hb + k4);
-    h1[c] = *(const float *)(Hb + i1);
-    h2[c] = *(const float *)(Hb + i2);
-    h3[c] = *(const float *)(Hb + i3);
+    h0[c] = htab<HLDS>(Hb, k4);
+    h1[c] = htab<HLDS>(Hb, i1);
+    h2[c] = htab<HLDS>(Hb, i2);
+    h3[c] = htab<HLDS>(Hb, i3);
   }
   /* then each cell's class and screen value, one cell at a time so that
    * only one cell's condition masks are live */
@@ -907,11 +965,29 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
 
   /* ---- D: forward sweep over this wave's rows w, w+W, ... ---- */
   uint32_t added = 0; /* rows [0, added) are in C */
-  for (uint32_t i = w; i < T1; i += W) {
-    rows_update(S, C, added, i + 1, true);
+  /* pairs [pe0, pe1) are the next update's (rows [added, i + 1)); the bins
+   * of its first pairs are in pb */
+  uint32_t pe0 = S.rowstart[0], pe1 = w < T1 ? S.rowstart[w + 1] : pe0;
+  uint32_t pb[kPreBins];
+  prefetch_bins(S, pe0, pe1, pb);
+  float lq[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+  /* the row constants of 64 steps at a time, lane l holding step 64 b + l's:
+   * one load per 64 rows instead of a global load latency per row */
+  uint32_t Kl = 0u;
+  float Al = 0.f;
+  for (uint32_t i = w, q = 0; i < T1; i += W, ++q) {
+    if ((q & 63u) == 0u) {
+      const uint32_t ir = i + lane * W;
+      Kl = ir < T1 ? J.K[ir] : 0u;
+      Al = ir < T1 ? J.A32[ir] : 0.f;
+    }
+    add_bins(S, C, pe0, pe1, pb);
     added = i + 1;
-    const uint32_t K = J.K[i];
-    const float A = J.A32[i];
+    const uint32_t inext = i + W;
+    uint32_t ne1 = 0u;
+    if (inext < T1) ne1 = S.rowstart[inext + 1];
+    const uint32_t K = (uint32_t)__builtin_amdgcn_readlane((int)Kl, (int)(q & 63u));
+    const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Al), (int)(q & 63u)));
     float lmin = __builtin_inff(); /* packed (value, cell) */
     uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
     /* REGCOLS: the row is one segment */
@@ -923,7 +999,13 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       if (REGCOLS) run = wave_excl_scan(tot);
       uint32_t kk[kSegCells], nn4[kSegCells];
       float xx[kSegCells];
-      segment_fast<REGCOLS>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+      segment_fast<HLDS, REGCOLS>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+      if (t0 == 0u && inext < T1) {
+        /* the next update's pairs: rows [i + 1, inext + 1) */
+        pe0 = pe1;
+        pe1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)ne1);
+        prefetch_bins(S, pe0, pe1, pb);
+      }
       run += tot;
       float smin = __builtin_inff();
 #pragma unroll
@@ -954,11 +1036,22 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
         }
       }
     }
-    lmin = wave_minf(lmin);
-    if (lane == 0) {
-      /* lower bound of the row's sum-class log p (subnormal sums: -inf) */
-      const float L = lmin - margin - pack_slack(lmin);
-      S.rowmin[i] = L < -708.f ? -__builtin_inff() : L;
+    /* row minima are reduced four rows at a time (rows i - 3W .. i) */
+    lq[0] = lq[1];
+    lq[1] = lq[2];
+    lq[2] = lq[3];
+    lq[3] = lmin;
+    if ((q & 3u) == 3u || inext >= T1) {
+      wave_minf_n<4>(lq);
+      if (lane == 0) {
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) {
+          if (q + r < 3u) continue; /* before the first row */
+          /* lower bound of the row's sum-class log p (subnormal sums: -inf) */
+          const float L = lq[r] - margin - pack_slack(lq[r]);
+          S.rowmin[i - (3u - r) * W] = L < -708.f ? -__builtin_inff() : L;
+        }
+      }
     }
   }
 
@@ -997,6 +1090,8 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     const uint32_t top = added - 1u - b0 * W; /* this batch's first (highest) row */
     const bool have = b0 + lane < nown;
     const float rm = have ? S.rowmin[top - lane * W] : __builtin_inff();
+    const uint32_t Kl = have ? J.K[top - lane * W] : 0u;
+    const float Al = have ? J.A32[top - lane * W] : 0.f;
     uint64_t done = 0;
 #pragma unroll 1
     for (;;) {
@@ -1008,8 +1103,8 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       rows_update(S, C, i + 1u, ctop, false);
       ctop = i + 1u;
       st.rows++;
-      const uint32_t K = J.K[i];
-      const float A = J.A32[i];
+      const uint32_t K = (uint32_t)__builtin_amdgcn_readlane((int)Kl, (int)l);
+      const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Al), (int)l));
       /* a row whose survivors overflow the list is flushed and rescanned from
        * the cell that did not fit (cells are visited in a fixed wave-uniform
        * order, ordinal 2 t0 + c), so every pass makes progress */
@@ -1026,7 +1121,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
           if (REGCOLS) run = wave_excl_scan(tot);
           uint32_t kk[kSegCells], nn4[kSegCells];
           float xx[kSegCells];
-          segment_fast<REGCOLS>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+          segment_fast<HLDS, REGCOLS>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
           run += tot;
           /* cheap first test: the first-term bound alone; cells whose bound
            * is below -708 (possibly subnormal or zero sums) always pass */
@@ -1041,7 +1136,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
           for (uint32_t c = 0; c < kSegCells; ++c) stot += (uint32_t)__popcll(__ballot(((surv >> c) & 1u) != 0u));
           /* make room first; only a segment with more survivors than the list
            * holds goes through the flush-and-resume path below */
-          if (n0 + stot > (uint32_t)kCand0Cap) flush_first<NT>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st);
+          if (n0 + stot > (uint32_t)kCand0Cap) flush_first<NT, HLDS>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st);
 #pragma unroll
           for (uint32_t c = 0; c < kSegCells; ++c) {
             const bool f = ((surv >> c) & 1u) != 0u && !again;
@@ -1060,7 +1155,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       }
     }
   }
-  if (n0) flush_first<NT>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st);
+  if (n0) flush_first<NT, HLDS>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st);
   uint64_t tc3 = cycles();
   st.cyc[2] += tc3 - tc2;
   if (nc) flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
@@ -1159,8 +1254,9 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
                                                  int32_t permuted, dto_record *__restrict__ out,
                                                  int *__restrict__ err, uint32_t *__restrict__ ws,
                                                  uint32_t *__restrict__ counter) {
+  /* no static LDS here: the dynamic region then starts at address 0 */
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint32_t s_next;
+  uint32_t &s_next = *(uint32_t *)(smem + Lo.misc);
   constexpr int W = NT / 64;
   const Lds S = lds_view(smem, Lo, W);
   load_job_tables<NT, HLDS>(J, S);
@@ -1228,10 +1324,14 @@ static __global__ void k_reset_counter(uint32_t *counter, int *err) {
 template <int NT, bool HLDS, bool PL, bool REGCOLS>
 __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs, const WorkItem *__restrict__ items,
                                               uint32_t n_items, uint64_t seed, uint32_t *__restrict__ counter,
-                                              int *__restrict__ err, uint32_t *__restrict__ ws, uint32_t ws_stride) {
+                                              int *__restrict__ err, uint32_t *__restrict__ ws, uint32_t ws_stride,
+                                              uint32_t item_slot) {
+  /* no static LDS here: the dynamic region then starts at address 0.  The
+   * work item slot is the misc slot of the batch's largest layout, which
+   * every other job's layout ends before. */
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int W = NT / 64;
-  __shared__ uint32_t s_item;
+  uint32_t &s_item = *(uint32_t *)(smem + item_slot);
   uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * ws_stride;
   uint32_t cur = kNone;
   WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
