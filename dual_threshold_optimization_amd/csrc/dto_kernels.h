@@ -683,19 +683,26 @@ __device__ __forceinline__ bool exact_zero(const DevJob &J, uint32_t K, uint32_t
 template <int NT, bool HLDS>
 __device__ __forceinline__ void flush_first(const DevJob &J, const Lds &S, const char *Hb, uint2 *cand0, uint32_t &n0,
                                             uint2 *cand, uint32_t &nc, uint4 *c3, uint32_t &n3, BestKey &xb,
-                                            float &Uw, float &U2w, WaveStats &st) {
+                                            float &Uw, float &U2w, WaveStats &st, uint32_t top, uint32_t Kl,
+                                            float Al) {
+  constexpr uint32_t W = NT / 64;
   const uint32_t lane = lane_id(), N = J.N;
   wave_sync();
 #pragma unroll 1
   for (uint32_t r0 = 0; r0 < n0; r0 += 64) {
     bool f = false;
-    uint2 cd = make_uint2(0u, 0u);
-    if (r0 + lane < n0) {
-      cd = cand0[r0 + lane];
-      const uint32_t i = cd.x >> 16, j = cd.x & 0xFFFFu, k = cd.y;
-      const uint32_t K = J.K[i], n = S.coln[j];
+    const bool act = r0 + lane < n0;
+    uint2 cd = act ? cand0[r0 + lane] : make_uint2(top << 16, 0u);
+    /* every listed cell is in a row of the current batch (row top - l W is
+     * lane l's): the row constants come from that lane's registers */
+    const int src = (int)((top - (cd.x >> 16)) / W);
+    const uint32_t K = (uint32_t)__shfl((int)Kl, src);
+    const float A = __shfl(Al, src);
+    if (act) {
+      const uint32_t j = cd.x & 0xFFFFu, k = cd.y;
+      const uint32_t n = S.coln[j];
       if (k <= min(K, n)) {
-        const float x = (J.A32[i] + S.colb[j]) -
+        const float x = (A + S.colb[j]) -
                         ((htab<HLDS>(Hb, 4u * k) + htab<HLDS>(Hb, 4u * (K - k))) +
                          (htab<HLDS>(Hb, 4u * (n - k)) + htab<HLDS>(Hb, 4u * ((N - K) - (n - k)))));
         if (!(x + J.margin < -745.2f && exact_zero(J, K, n, k, x))) {
@@ -1136,7 +1143,8 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
           for (uint32_t c = 0; c < kSegCells; ++c) stot += (uint32_t)__popcll(__ballot(((surv >> c) & 1u) != 0u));
           /* make room first; only a segment with more survivors than the list
            * holds goes through the flush-and-resume path below */
-          if (n0 + stot > (uint32_t)kCand0Cap) flush_first<NT, HLDS>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st);
+          if (n0 + stot > (uint32_t)kCand0Cap)
+            flush_first<NT, HLDS>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st, top, Kl, Al);
 #pragma unroll
           for (uint32_t c = 0; c < kSegCells; ++c) {
             const bool f = ((surv >> c) & 1u) != 0u && !again;
@@ -1154,8 +1162,9 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
         /* again: the next pass starts by flushing the full list */
       }
     }
+    /* the list holds rows of this batch only */
+    if (n0) flush_first<NT, HLDS>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st, top, Kl, Al);
   }
-  if (n0) flush_first<NT, HLDS>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st);
   uint64_t tc3 = cycles();
   st.cyc[2] += tc3 - tc2;
   if (nc) flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
