@@ -502,12 +502,13 @@ __device__ __forceinline__ float tail_lower_log(uint32_t N, uint32_t K, uint32_t
  * terms in FP32 (fewer once the rest is below 2^-16 S), then a geometric
  * bound of the rest (the ratio only falls with i).  Accumulated relative
  * error of the products and the sum is at most 9 (m+1) 2^-24; hi is +inf
- * when the terms are still growing at the cut. */
+ * when the terms are still growing at the cut.  The sum also stops once it
+ * passes s_stop (both ends stay valid bounds). */
 struct TailBracket {
   float lo, hi;
 };
 __device__ __forceinline__ TailBracket tail_bracket(uint32_t N, uint32_t K, uint32_t n, uint32_t k,
-                                                    uint32_t mmax) {
+                                                    uint32_t mmax, float s_stop) {
   const uint32_t mx = K < n ? K : n;
   float S = 1.f, P = 1.f;
   uint32_t i = k, m = 0;
@@ -517,7 +518,8 @@ __device__ __forceinline__ TailBracket tail_bracket(uint32_t N, uint32_t K, uint
     S += P;
     ++i;
     ++m;
-    if (P > 1e30f) break;
+    /* past s_stop the caller needs no more than the lower end */
+    if (P > 1e30f || S > s_stop) break;
     /* the geometric bound of the rest (ratios only fall) is below 2^-16 S */
     if (r < 0.999f && P * r <= S * (1.f - r) * 0x1p-16f) break;
   }
@@ -537,7 +539,7 @@ __device__ __forceinline__ TailBracket tail_bracket(uint32_t N, uint32_t K, uint
  * cell, from its FP32 first-term log x. */
 __device__ __forceinline__ float screen_upper(const DevJob &J, uint32_t K, uint32_t n, uint32_t k, float x) {
   const uint32_t mx = K < n ? K : n;
-  const TailBracket tb = tail_bracket(J.N, K, n, k, 64);
+  const TailBracket tb = tail_bracket(J.N, K, n, k, 64, __builtin_inff());
   if (!(tb.hi < __builtin_inff())) return 1e-6f; /* p <= 1 (+ rounding) */
   float U = x + J.margin + __logf(tb.hi) + 1e-4f;
   if (U < -690.f) U = fmaxf(U, __logf((float)(mx - k + 1)) - 744.44f) + 0.7f; /* subnormal term rounding */
@@ -615,7 +617,10 @@ __device__ __forceinline__ void flush_candidates(const DevJob &J, const Lds &S, 
       const uint32_t K = J.K[i], n = S.coln[j];
       const uint32_t mx = K < n ? K : n;
       const double x64 = first_exponent(J, K, n, k, J.D[j]);
-      const TailBracket tb = tail_bracket(J.N, K, n, k, 1024);
+      /* a sum above s_stop already puts the lower end above U2w (the
+       * 2e-3 covers the FP32 rounding of the exponent and eps <= 5.5e-4) */
+      const float s_stop = __expf(fminf(U2w - (float)x64 + 2e-3f, 80.f));
+      const TailBracket tb = tail_bracket(J.N, K, n, k, 1024, s_stop);
       const double lo = x64 + (double)__logf(tb.lo) - 1e-4;
       L2[r] = lo < -708.0 ? -__builtin_inff() : (float)(lo - 1e-4);
       if (tb.hi < __builtin_inff()) {
