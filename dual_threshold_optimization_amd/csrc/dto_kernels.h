@@ -205,23 +205,25 @@ __device__ __forceinline__ float htab(const char *Hb, uint32_t off) {
   return *(const float *)(Hb + off);
 }
 
-/* Copies n floats global -> LDS with 16-byte loads (both 16-byte aligned). */
+/* Copies n floats global -> LDS (both 16-byte aligned) with LDS DMA
+ * (global_load_lds_dwordx4): lane l of a wave writes 16 bytes at the wave's
+ * LDS base + 16 l, and the data never passes through VGPRs.  Lanes past the
+ * end are masked off, so nothing beyond dst[n) is written.  Ends with the
+ * wave's loads complete; the caller's barrier publishes them. */
 template <int NT>
 __device__ __forceinline__ void copy_table(float *dst, const float *src, uint32_t n) {
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   const uint32_t n4 = n / 4;
   const float4 *s4 = reinterpret_cast<const float4 *>(src);
   float4 *d4 = reinterpret_cast<float4 *>(dst);
-  for (uint32_t t0 = tid; t0 < n4; t0 += NT * 4) {
-    float4 v[4];
-#pragma unroll
-    for (uint32_t u = 0; u < 4; ++u)
-      if (t0 + u * NT < n4) v[u] = s4[t0 + u * NT];
-#pragma unroll
-    for (uint32_t u = 0; u < 4; ++u)
-      if (t0 + u * NT < n4) d4[t0 + u * NT] = v[u];
+  for (uint32_t c0 = w * 64u; c0 < n4; c0 += NT) {
+    const uint32_t t = c0 + lane;
+    if (t < n4)
+      __builtin_amdgcn_global_load_lds((const void *)(s4 + t), (__attribute__((address_space(3))) void *)(d4 + c0),
+                                       16, 0, 0);
   }
   for (uint32_t t = n4 * 4 + tid; t < n; t += NT) dst[t] = src[t];
+  __builtin_amdgcn_s_waitcnt(0);
 }
 
 /* Per-job tables that stay in LDS while a workgroup works on that job. */
