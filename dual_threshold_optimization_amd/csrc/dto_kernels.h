@@ -539,9 +539,10 @@ __device__ __forceinline__ TailBracket tail_bracket(uint32_t N, uint32_t K, uint
 
 /* Upper bound of the log of the reference's computed sum for a sum-class
  * cell, from its FP32 first-term log x. */
-__device__ __forceinline__ float screen_upper(const DevJob &J, uint32_t K, uint32_t n, uint32_t k, float x) {
+__device__ __forceinline__ float screen_upper(const DevJob &J, uint32_t K, uint32_t n, uint32_t k, float x,
+                                              uint32_t mmax = 64, float s_stop = __builtin_inff()) {
   const uint32_t mx = K < n ? K : n;
-  const TailBracket tb = tail_bracket(J.N, K, n, k, 64, __builtin_inff());
+  const TailBracket tb = tail_bracket(J.N, K, n, k, mmax, s_stop);
   if (!(tb.hi < __builtin_inff())) return 1e-6f; /* p <= 1 (+ rounding) */
   float U = x + J.margin + __logf(tb.hi) + 1e-4f;
   if (U < -690.f) U = fmaxf(U, __logf((float)(mx - k + 1)) - 744.44f) + 0.7f; /* subnormal term rounding */
@@ -1069,8 +1070,25 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     }
   }
 
-  float Urun = 1e-6f; /* every p <= 1 (+ rounding): log p <= 1e-6 always */
-  if (bx < 1e30f) Urun = fminf(Urun, screen_upper(J, bK, bn, bk, bx + pack_slack(bx)));
+  /* U*: every p <= 1 (+ rounding), so log p <= 1e-6 always.  Each lane
+   * brackets its best cell's tail with 16 ratio steps, which settles the
+   * fast-falling tails of small-p cells; the wave minimum U1 of those bounds
+   * is valid.  Only lanes still unsettled and with a screen value below U1
+   * (a lane's bound is never below its screen value) go on to 64 steps, and
+   * stop as soon as their sum shows they cannot get below U1: a lane whose
+   * terms keep growing (a cell below the mode) no longer sets the length of
+   * the wave's loop. */
+  float Urun = 1e-6f;
+  {
+    const float xs = bx + pack_slack(bx);
+    float r1 = 1e-6f;
+    if (bx < 1e30f) r1 = screen_upper(J, bK, bn, bk, xs, 16);
+    const bool open = bx < 1e30f && r1 == 1e-6f; /* 1e-6: not settled in 16 steps */
+    const float U1 = wave_minf(fminf(r1, 1e-6f));
+    Urun = U1;
+    if (open && xs < U1)
+      Urun = fminf(Urun, screen_upper(J, bK, bn, bk, xs, 64, __expf(fminf(U1 - xs, 80.f))));
+  }
   Urun = wave_minf(Urun);
   zb = wave_zero(zb);
   if (zb.ij != kNone) Urun = -__builtin_inff();
