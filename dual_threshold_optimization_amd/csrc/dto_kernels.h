@@ -568,6 +568,58 @@ struct WaveStats {
 
 __device__ __forceinline__ uint64_t cycles() { return clock64(); }
 
+/* dto_tail_exact for one cell with the whole wave (wave-uniform arguments,
+ * result in every lane): lane l evaluates term k + 64 b + l of block b, so a
+ * block's table loads and exps run side by side; the terms are then added
+ * in ascending order, one at a time, with the same early exit, so the sum
+ * is bit-identical to dto_tail_exact's.  Used when a wave has only a few
+ * exact cells, where one lane per cell would leave the wave (and the
+ * workgroup, at the next barrier) waiting on that lane's serial terms. */
+__device__ __forceinline__ double wave_tail_exact(const DevJob &J, const Lds &S, uint32_t K, uint32_t n, uint32_t k,
+                                                  double d, uint32_t *terms) {
+  *terms = 0;
+  const uint32_t N = J.N;
+  if (k == 0) return 1.0;
+  const uint32_t x = k - 1;
+  const uint32_t mn = (K + n > N) ? K + n - N : 0;
+  const uint32_t mx = K < n ? K : n;
+  if (x < mn) return 1.0;
+  if (x >= mx) return 0.0;
+  const double *LF = J.LF;
+  const double lfK = LF[K], lfNK = LF[N - K];
+  const uint32_t NK = N - K, lane = lane_id();
+  double acc = 0.0;
+  for (uint32_t i0 = k; i0 <= mx; i0 += 64) {
+    const uint32_t i = i0 + lane;
+    double t = 0.0;
+    bool fall = false; /* rho_i <= 1 - 2^-20, as num <= den - ceil(den / 2^20) */
+    if (i <= mx) {
+      const double lb1 = (lfK - LF[i]) - LF[K - i];
+      const double lb2 = (lfNK - LF[n - i]) - LF[NK - (n - i)];
+      t = dto_exp((lb1 + lb2) - d, S.tab);
+      const uint64_t num = (uint64_t)(K - i) * (uint64_t)(n - i);
+      const uint64_t den = (uint64_t)(i + 1) * (uint64_t)(NK - n + i + 1);
+      fall = num <= den - ((den + 0xFFFFFull) >> 20);
+    }
+    const uint64_t fm = __ballot(fall);
+    const uint32_t cnt = min(64u, mx - i0 + 1u);
+    const int2 tv = *reinterpret_cast<const int2 *>(&t);
+    for (uint32_t l = 0; l < cnt; ++l) {
+      int2 v;
+      v.x = __builtin_amdgcn_readlane(tv.x, (int)l);
+      v.y = __builtin_amdgcn_readlane(tv.y, (int)l);
+      const double tl = *reinterpret_cast<const double *>(&v);
+      acc = acc + tl;
+      if (tl * 0x1p56 <= acc && ((fm >> l) & 1ull)) {
+        *terms = i0 + l + 1 - k;
+        return acc;
+      }
+    }
+  }
+  *terms = mx + 1 - k;
+  return acc;
+}
+
 /* Exact FP64 evaluation (bit-identical to the reference) of the tier-3 cells
  * whose lower bound is <= thr. */
 __device__ __forceinline__ void flush_exact(const DevJob &J, const Lds &S, uint4 *c3, uint32_t &n3, float thr, BestKey &xb,
@@ -576,15 +628,33 @@ __device__ __forceinline__ void flush_exact(const DevJob &J, const Lds &S, uint4
   wave_sync();
   BestKey lb = {__builtin_inf(), 0u, kNone};
   uint32_t cells = 0;
-  for (uint32_t c = lane; c < n3; c += 64) {
-    const uint4 cd = c3[c];
-    if (__uint_as_float(cd.z) > thr) continue;
-    const uint32_t i = cd.x >> 16, j = cd.x & 0xFFFFu, k = cd.y;
-    uint32_t nt = 0;
-    double p = dto_tail_exact(J.LF, J.N, J.K[i], S.coln[j], k, J.D[j], S.tab, &nt);
-    st.terms += nt;
-    ++cells;
-    take_better(lb, p, k, cd.x);
+  static_assert(kCand3Cap <= 64, "one lane per listed cell");
+  const uint32_t nex = (uint32_t)__popcll(__ballot(lane < n3 && __uint_as_float(c3[lane].z) <= thr));
+  if (nex <= 4u) {
+    /* few cells: each with the whole wave */
+    for (uint32_t c = 0; c < n3; ++c) {
+      const uint4 cd = c3[c];
+      if (__uint_as_float(cd.z) > thr) continue;
+      const uint32_t i = cd.x >> 16, j = cd.x & 0xFFFFu, k = cd.y;
+      uint32_t nt = 0;
+      const double p = wave_tail_exact(J, S, J.K[i], S.coln[j], k, J.D[j], &nt);
+      if (lane == 0) {
+        st.terms += nt;
+        ++cells;
+      }
+      take_better(lb, p, k, cd.x);
+    }
+  } else {
+    for (uint32_t c = lane; c < n3; c += 64) {
+      const uint4 cd = c3[c];
+      if (__uint_as_float(cd.z) > thr) continue;
+      const uint32_t i = cd.x >> 16, j = cd.x & 0xFFFFu, k = cd.y;
+      uint32_t nt = 0;
+      double p = dto_tail_exact(J.LF, J.N, J.K[i], S.coln[j], k, J.D[j], S.tab, &nt);
+      st.terms += nt;
+      ++cells;
+      take_better(lb, p, k, cd.x);
+    }
   }
   for (int off = 32; off >= 1; off >>= 1) cells += __shfl_xor(cells, off);
   st.cells += cells;
