@@ -55,7 +55,7 @@ struct LdsLayout {
   uint32_t cand3;    /* uint4[W][kCand3Cap] tier-2 survivors */
   uint32_t exptab;   /* u64[256] */
   uint32_t red;      /* per-wave reduction slots */
-  uint32_t misc;     /* u32[4]: the workgroup's current work item */
+  uint32_t misc;     /* u32[4]: [0] the workgroup's current work item, [2] the shared bound key */
   uint32_t total;
   uint32_t T2w_pad;
   uint32_t h_in_lds;

@@ -168,6 +168,7 @@ struct Lds {
   float *redU;
   BestKey *redB; /* [3][W]: zero class, one class, evaluated */
   uint32_t *redS;
+  int *ushare; /* the reverse sweep's best upper bound of the minimum log p, as an ordered key */
   float *redU2;
 };
 
@@ -191,8 +192,22 @@ __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo
   s.redU = (float *)(smem + Lo.red);
   s.redB = (BestKey *)(smem + Lo.red + 16 * W);
   s.redS = (uint32_t *)(smem + Lo.red + 16 * W + 3 * W * sizeof(BestKey));
+  s.ushare = (int *)(smem + Lo.misc) + 2;
   s.redU2 = (float *)(smem + Lo.red + 16 * W + 3 * W * sizeof(BestKey) + 4 * W);
   return s;
+}
+
+/* Order-preserving map between floats and ints (an involution), so that an
+ * integer atomic minimum keeps the smallest float. */
+__device__ __forceinline__ int fkey(float f) {
+  const int b = __float_as_int(f);
+  return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
+__device__ __forceinline__ float fkey_value(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
+
+/* Shares a wave's upper bound of the minimum log p with the other waves. */
+__device__ __forceinline__ void share_bound(const Lds &S, float U) {
+  if (lane_id() == 0) atomicMin(S.ushare, fkey(U));
 }
 
 /* One screen table entry by byte offset.  In LDS the table starts the
@@ -707,6 +722,7 @@ __device__ __forceinline__ void flush_candidates(const DevJob &J, const Lds &S, 
   }
   U2w = fminf(U2w, wave_minf(U2min));
   Uw = fminf(Uw, U2w);
+  share_bound(S, U2w);
   /* survivors move to the tier-3 list half a wave at a time: at most 32
    * entries per push, so a push after a flush always fits the list */
   static_assert(kCand3Cap >= 32, "a half-wave push must fit an empty tier-3 list");
@@ -724,6 +740,7 @@ __device__ __forceinline__ void flush_candidates(const DevJob &J, const Lds &S, 
           flush_exact(J, S, c3, n3, U2w, xb, st);
           U2w = fminf(U2w, log_bound_of(xb));
           Uw = fminf(Uw, U2w);
+          share_bound(S, U2w);
         }
         if (fh) {
           const uint2 cd = cand[c];
@@ -1168,6 +1185,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     S.redU[w] = Urun;
     S.redB[w] = BestKey{0.0, zb.k, zb.ij};
   }
+  if (tid == 0) *S.ushare = fkey(__builtin_inff());
   __syncthreads();
   float Ustar = __builtin_inff();
   for (uint32_t v = 0; v < W; ++v) Ustar = fminf(Ustar, S.redU[v]);
@@ -1197,6 +1215,10 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     uint64_t done = 0;
 #pragma unroll 1
     for (;;) {
+      /* bounds found by the other waves apply here too */
+      const float us = fkey_value(*(volatile int *)S.ushare);
+      Uw = fminf(Uw, us);
+      U2w = fminf(U2w, us);
       const uint64_t m = __ballot(rm <= Uw) & ~done;
       if (!m) break;
       const uint32_t l = (uint32_t)__builtin_ctzll(m);
