@@ -307,6 +307,7 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
   volatile uint32_t *vpairs = gpairs; /* global staging: bypass the per-CU cache on the way back */
   for (uint32_t t = tid; t <= T1; t += NT) S.rowstart[t] = 0;
   for (uint32_t t = tid; t < W * T2wp; t += NT) S.cw[t] = 0;
+  if (tid == 0) *S.ushare = fkey(__builtin_inff()); /* published after the build's barriers */
   /* A: list1 feature m sits at the position k with idx1[k] == m (loads of
    * kU positions in flight per thread) */
   constexpr uint32_t kU = 16;
@@ -1185,10 +1186,13 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     S.redU[w] = Urun;
     S.redB[w] = BestKey{0.0, zb.k, zb.ij};
   }
-  if (tid == 0) *S.ushare = fkey(__builtin_inff());
-  __syncthreads();
-  float Ustar = __builtin_inff();
-  for (uint32_t v = 0; v < W; ++v) Ustar = fminf(Ustar, S.redU[v]);
+  /* no barrier here: each wave goes on to its own rows' reverse sweep as
+   * soon as its forward sweep is done, starting from its own bound, and
+   * takes up the other waves' bounds through the shared minimum as they
+   * arrive (the reverse sweep reads only this wave's counts, lists and row
+   * bounds) */
+  share_bound(S, Urun);
+  const float Ustar = fminf(Urun, fkey_value(*(volatile int *)S.ushare));
   float Uw = Ustar;  /* tier-1 threshold (wave, tightens as tier 2 runs) */
   float U2w = Ustar; /* best tier-2 upper end seen by this wave */
 
@@ -1291,7 +1295,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   if (lane == 0) S.redU2[w] = fminf(U2w, log_bound_of(xb));
   __syncthreads();
   uint64_t tc5 = cycles();
-  float U2s = Ustar;
+  float U2s = fkey_value(*(volatile int *)S.ushare); /* every wave's bounds are in */
   for (uint32_t v = 0; v < W; ++v) U2s = fminf(U2s, S.redU2[v]);
   if (n3) flush_exact(J, S, c3, n3, U2s, xb, st);
   st.cyc[4] += cycles() - tc5;
@@ -1313,7 +1317,9 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   BestKey b = {__builtin_inf(), 0u, kNone};
   if (tid == 0) {
     for (uint32_t v = 0; v < 3 * W; ++v) take_better(b, S.redB[v].p, S.redB[v].k, S.redB[v].ij);
-    if (b.p > 0.0 && b.p < 1.0 && Ustar < 1e30f) st.gap += (uint32_t)fmaxf(0.f, (Ustar - (float)log(b.p)) * 1000.f);
+    float U0 = __builtin_inff(); /* the forward sweeps' bound */
+    for (uint32_t v = 0; v < W; ++v) U0 = fminf(U0, S.redU[v]);
+    if (b.p > 0.0 && b.p < 1.0 && U0 < 1e30f) st.gap += (uint32_t)fmaxf(0.f, (U0 - (float)log(b.p)) * 1000.f);
   }
   __syncthreads();
   return b;
