@@ -169,7 +169,6 @@ struct Lds {
   BestKey *redB; /* [3][W]: zero class, one class, evaluated */
   uint32_t *redS;
   int *ushare; /* the reverse sweep's best upper bound of the minimum log p, as an ordered key */
-  float *redU2;
 };
 
 __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo, int W) {
@@ -193,7 +192,6 @@ __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo
   s.redB = (BestKey *)(smem + Lo.red + 16 * W);
   s.redS = (uint32_t *)(smem + Lo.red + 16 * W + 3 * W * sizeof(BestKey));
   s.ushare = (int *)(smem + Lo.misc) + 2;
-  s.redU2 = (float *)(smem + Lo.red + 16 * W + 3 * W * sizeof(BestKey) + 4 * W);
   return s;
 }
 
@@ -1291,13 +1289,16 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   if (nc) flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
   uint64_t tc4 = cycles();
   st.cyc[3] += tc4 - tc3;
-  /* tier 3: exact FP64 for cells whose bracket reaches the best bracket of any wave */
-  if (lane == 0) S.redU2[w] = fminf(U2w, log_bound_of(xb));
-  __syncthreads();
+  /* tier 3: exact FP64 for cells whose bracket reaches the best bound this
+   * wave knows of (its own, and what the other waves have shared so far: no
+   * barrier, a looser bound only evaluates more cells) */
   uint64_t tc5 = cycles();
-  float U2s = fkey_value(*(volatile int *)S.ushare); /* every wave's bounds are in */
-  for (uint32_t v = 0; v < W; ++v) U2s = fminf(U2s, S.redU2[v]);
-  if (n3) flush_exact(J, S, c3, n3, U2s, xb, st);
+  {
+    const float U2o = fminf(U2w, log_bound_of(xb));
+    share_bound(S, U2o);
+    const float U2s = fminf(U2o, fkey_value(*(volatile int *)S.ushare));
+    if (n3) flush_exact(J, S, c3, n3, U2s, xb, st);
+  }
   st.cyc[4] += cycles() - tc5;
   if (lane == 0) S.redB[2 * W + w] = xb;
   __syncthreads();
