@@ -194,6 +194,24 @@ def test_screened_equals_plain_genome_20k(dto, gpu):
     assert rs.empirical_pvalue == rp.empirical_pvalue
 
 
+def test_screened_equals_plain_many_permutations(dto, gpu):
+    """The screened kernel's waves share bounds and run their sweeps and the
+    exact tier without barriers between them; thousands of permutations, on
+    both permutation paths, must still give exactly the plain records."""
+    from dual_threshold_optimization_amd import workloads as W
+    ids1, r1, ids2, r2, bg = W.genome_20k()
+    job = dto.Job.from_lists(ids1, r1, ids2, r2, background=bg, drop_outside_background=True)
+    P = 2048
+    rs, ps = job.run(P, seed=17, mode="screened")
+    rp, pp = job.run(P, seed=17, mode="plain")
+    assert [rec_tuple(x) for x in ps] == [rec_tuple(x) for x in pp]
+    assert (rs.n_le, rs.empirical_pvalue) == (rp.n_le, rp.empirical_pvalue)
+    i1, i2 = W.explicit_permutations(1024, job.len1, job.len2, 29)
+    _, es = job.run(1024, idx1=i1, idx2=i2, mode="screened")
+    _, ep = job.run(1024, idx1=i1, idx2=i2, mode="plain")
+    assert [rec_tuple(x) for x in es] == [rec_tuple(x) for x in ep]
+
+
 def test_dense_thresholds_vs_oracle(dto, O, gpu):
     """Custom threshold grids wider than one register segment per lane
     (column constants read from LDS), against the oracle with the same
