@@ -253,12 +253,12 @@ __device__ __forceinline__ void load_job_tables(const DevJob &J, const Lds &S) {
 
 /* Visits the Philox permutation of one list in rank order, fn(k, idx[k])
  * for every k < n, spread over the workgroup.  The domain [0, 2^b) goes in
- * chunks of NT * 8 consecutive elements, each thread evaluating 8
+ * chunks of NT * 16 consecutive elements, each thread evaluating 16
  * consecutive ones; a workgroup prefix count (wave scan + per-wave totals in
  * wsum[W]) ranks those that fall below n.  Every thread runs every chunk. */
 template <int NT, typename Fn>
 __device__ __forceinline__ void shuffle_visit(const Shuffle &s, uint32_t *wsum, Fn fn) {
-  constexpr uint32_t kU = 8, W = NT / 64;
+  constexpr uint32_t kU = 16, W = NT / 64;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   uint32_t carried = 0;
   for (uint32_t base = 0; base < s.dom; base += NT * kU) {
