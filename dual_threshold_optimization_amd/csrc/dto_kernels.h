@@ -617,12 +617,10 @@ __device__ __forceinline__ double wave_tail_exact(const DevJob &J, const Lds &S,
     }
     const uint64_t fm = __ballot(fall);
     const uint32_t cnt = min(64u, mx - i0 + 1u);
-    const int2 tv = *reinterpret_cast<const int2 *>(&t);
+    const int tlo = __double2loint(t), thi = __double2hiint(t);
     for (uint32_t l = 0; l < cnt; ++l) {
-      int2 v;
-      v.x = __builtin_amdgcn_readlane(tv.x, (int)l);
-      v.y = __builtin_amdgcn_readlane(tv.y, (int)l);
-      const double tl = *reinterpret_cast<const double *>(&v);
+      const double tl =
+          __hiloint2double(__builtin_amdgcn_readlane(thi, (int)l), __builtin_amdgcn_readlane(tlo, (int)l));
       acc = acc + tl;
       if (tl * 0x1p56 <= acc && ((fm >> l) & 1ull)) {
         *terms = i0 + l + 1 - k;
@@ -644,7 +642,7 @@ __device__ __forceinline__ void flush_exact(const DevJob &J, const Lds &S, uint4
   uint32_t cells = 0;
   static_assert(kCand3Cap <= 64, "one lane per listed cell");
   const uint32_t nex = (uint32_t)__popcll(__ballot(lane < n3 && __uint_as_float(c3[lane].z) <= thr));
-  if (nex <= 4u) {
+  if (nex <= 8u) {
     /* few cells: each with the whole wave */
     for (uint32_t c = 0; c < n3; ++c) {
       const uint4 cd = c3[c];
