@@ -1224,7 +1224,13 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       const uint32_t l = (uint32_t)__builtin_ctzll(m);
       done |= (2ull << l) - 1ull; /* rows at or above this one are passed */
       const uint32_t i = top - l * W;
-      rows_update(S, C, i + 1u, ctop, false);
+      if (S.rowstart[i + 1u] - S.rowstart[0] < S.rowstart[ctop] - S.rowstart[i + 1u]) {
+        /* fewer pairs below than between: rebuild the counts from zero */
+        for (uint32_t t = lane; t < Lo.T2w_pad; t += 64u) C[t] = 0u;
+        rows_update(S, C, 0u, i + 1u, true);
+      } else {
+        rows_update(S, C, i + 1u, ctop, false);
+      }
       ctop = i + 1u;
       st.rows++;
       const uint32_t K = (uint32_t)__builtin_amdgcn_readlane((int)Kl, (int)l);
