@@ -152,6 +152,7 @@ int upload_job(dto_job *j) {
   d.T2 = (uint32_t)h.thr2.size();
   d.T2w = (d.T2 + 1) / 2;
   d.dup = h.dup ? 1u : 0u;
+  d.kle = h.kle ? 1u : 0u;
   d.margin = h.margin;
   d.bin1 = (const uint16_t *)(b + o_bin1);
   d.bin2 = (const uint16_t *)(b + o_bin2);

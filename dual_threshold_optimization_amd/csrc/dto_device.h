@@ -20,6 +20,7 @@ struct DevJob {
   uint32_t T1, T2;   /* threshold counts */
   uint32_t T2w;      /* ceil(T2 / 2): column counts are packed two per u32 */
   uint32_t dup;      /* 1: some list2 entry joins several list1 entries (CSR) */
+  uint32_t kle;      /* 1: every overlap k_ij <= K_i (no cell has an empty tail) */
   float margin;      /* bound on |screen log-term - exact log-term| (nats) */
   const uint16_t *bin1;     /* [N1] threshold bin of sorted list1 position (T1 = none) */
   const uint16_t *bin2;     /* [N2] */

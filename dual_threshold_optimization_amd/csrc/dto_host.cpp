@@ -252,6 +252,17 @@ int HostJob::derive() {
     }
   }
   join_off[N2] = (uint32_t)join_pos.size();
+  /* with one partner per list2 entry and no list1 entry shared by two list2
+   * entries, the pairs with a <= i are distinct list1 entries: k_ij <= K_i */
+  kle = !dup;
+  if (kle) {
+    std::vector<uint8_t> used(l1.ids.size(), 0);
+    for (size_t q = 0; q < N2 && kle; ++q)
+      if (join[q] >= 0) {
+        if (used[(size_t)join[q]]) kle = false;
+        used[(size_t)join[q]] = 1;
+      }
+  }
 
   /* ln n! table, exactly as statrs evaluates each call */
   LF.resize(N + 1);

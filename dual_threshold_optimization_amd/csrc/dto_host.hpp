@@ -38,6 +38,7 @@ struct HostJob {
   std::vector<int32_t> join;
   std::vector<uint32_t> join_off, join_pos;
   bool dup = false;
+  bool kle = false; /* every overlap k_ij <= K_i: no list1 entry pairs with two list2 entries */
   std::vector<uint32_t> K, n;
   std::vector<double> LF, D;
   std::vector<float> H32, A32, B32;

@@ -882,10 +882,12 @@ __device__ __forceinline__ uint32_t read_segment(const uint32_t *C, uint32_t w0,
  * (k == 0, or k <= K+n-N, i.e. n-k >= N-K) or no column get x ~ +3e38;
  * cells with k > K (p == 0: an empty tail; k <= n always, since k counts
  * entries of the column's set) get x ~ -3e38, so that a segment minimum
- * below the underflow bound flags every cell that may have p == 0.  Hb is
+ * below the underflow bound flags every cell that may have p == 0 (ZC
+ * false: the job has no such cells, see DevJob.kle, and the test is left
+ * out).  Hb is
  * the byte base of the H table; indices are byte offsets (4x), and the two
  * that can wrap for cells of the other classes saturate at 0. */
-template <bool HLDS, bool REGCOLS>
+template <bool HLDS, bool REGCOLS, bool ZC = true>
 __device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, const char *Hb, const uint32_t *wv,
                                              uint32_t nseg, uint32_t w0, uint32_t run, uint32_t K, float A,
                                              const ColRegs &cr, uint32_t *kk, uint32_t *nn4, float *xx) {
@@ -925,7 +927,7 @@ __device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, cons
   for (uint32_t c = 0; c < kSegCells; ++c) {
     __builtin_amdgcn_sched_barrier(0);
     const bool one = kk[c] == 0u || d2[c] >= NK4;
-    const bool zero = kk[c] > K;
+    const bool zero = ZC && kk[c] > K; /* ZC false: the job has no k > K cells */
     const float b = zero ? -kNoCell : (one ? kNoCell : bc[c]);
     xx[c] = (A + b) - ((h0[c] + h1[c]) + (h2[c] + h3[c]));
   }
@@ -1098,7 +1100,10 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       if (REGCOLS) run = wave_excl_scan(tot);
       uint32_t kk[kSegCells], nn4[kSegCells];
       float xx[kSegCells];
-      segment_fast<HLDS, REGCOLS>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+      if (J.kle)
+        segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+      else
+        segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
       if (t0 == 0u && inext < T1) {
         /* the next update's pairs: rows [i + 1, inext + 1) */
         pe0 = pe1;
@@ -1251,7 +1256,10 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
           if (REGCOLS) run = wave_excl_scan(tot);
           uint32_t kk[kSegCells], nn4[kSegCells];
           float xx[kSegCells];
-          segment_fast<HLDS, REGCOLS>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+          if (J.kle)
+            segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+          else
+            segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
           run += tot;
           /* cheap first test: the first-term bound alone; cells whose bound
            * is below -708 (possibly subnormal or zero sums) always pass */

@@ -124,6 +124,27 @@ def test_optimize_random_vs_oracle(dto, O, mode, gpu):
             assert rec_tuple(got) == rec_tuple(want), (case, t, n)
 
 
+def test_list2_duplicates_vs_oracle(dto, O, gpu):
+    """Duplicate ids in list 2 only: two list2 entries pair with one list1
+    entry, so overlaps can exceed K (empty tails, p == 0).  The screen keeps
+    its k > K test for such jobs; results match the oracle."""
+    rng = np.random.default_rng(23)
+    for case in range(24):
+        n = int(rng.integers(6, 300))
+        ids1, r1, ids2, r2 = random_lists(rng, n, tie_frac=0.2 * (case % 3))
+        for q in range(0, len(ids2) - 1, 3 + case % 4):
+            ids2[q + 1] = ids2[q]
+        N = len(set(ids1) | set(ids2)) + int(rng.integers(0, 3))
+        job = dto.Job.with_population(ids1, r1, ids2, r2, N)
+        l1, l2 = _oracle_pair(O, ids1, r1, ids2, r2)
+        for t in range(3):
+            p1 = rng.permutation(len(ids1)) if t else None
+            p2 = rng.permutation(len(ids2)) if t else None
+            want = O.optimize(l1, l2, N, p1, p2)
+            for mode in ("screened", "plain"):
+                assert rec_tuple(job.optimize(p1, p2, mode=mode)) == rec_tuple(want), (case, t, n, mode)
+
+
 @pytest.mark.parametrize("mode", ["plain", "screened"])
 def test_run_plain_6k_explicit(dto, O, mode, gpu):
     from dual_threshold_optimization_amd import workloads as W
