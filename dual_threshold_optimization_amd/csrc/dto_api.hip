@@ -64,7 +64,7 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
   const uint32_t N1 = (uint32_t)h.l1.ids.size(), N2 = (uint32_t)h.l2.ids.size();
   const uint32_t T1 = (uint32_t)h.thr1.size(), T2 = (uint32_t)h.thr2.size();
   const uint32_t T2w = (T2 + 1) / 2;
-  L.T2w_pad = (T2w + 3u) & ~3u;
+  L.T2w_pad = (T2w + 4u) & ~3u; /* at least one zero padding word */
   /* the screen table (or, without it, the pair staging) sits at offset 0 so
    * that table reads address it with the scaled index alone */
   const uint32_t hb = with_h ? 4u * (uint32_t)(h.N + 1) : 0u, pb = 4u * std::max(N2, 1u);

@@ -857,18 +857,20 @@ struct ColRegs {
   float b[kSegCells];
 };
 
-/* Reads a segment's count words (all loads issued together, indices
- * clamped into the wave's row of T2w words) and returns the lane's count
- * total. */
-__device__ __forceinline__ uint32_t read_segment(const uint32_t *C, uint32_t w0, uint32_t nseg, uint32_t T2w,
+/* Reads a segment's count words (all loads issued together) and returns
+ * the lane's count total.  Indices are clamped to the wave's last padding
+ * word, which is always zero (the padding is at least one word and no pair
+ * reaches it), so words past the row need no per-lane mask; only words past
+ * the segment (u >= nseg, the same for every lane) are masked. */
+__device__ __forceinline__ uint32_t read_segment(const uint32_t *C, uint32_t w0, uint32_t nseg, uint32_t last,
                                                  uint32_t *wv) {
   uint32_t raw[kSegWords];
 #pragma unroll
-  for (uint32_t u = 0; u < kSegWords; ++u) raw[u] = C[min(w0 + u, T2w - 1u)];
+  for (uint32_t u = 0; u < kSegWords; ++u) raw[u] = C[min(w0 + u, last)];
   uint32_t tot = 0;
 #pragma unroll
   for (uint32_t u = 0; u < kSegWords; ++u) {
-    wv[u] = (u < nseg && w0 + u < T2w) ? raw[u] : 0u;
+    wv[u] = u < nseg ? raw[u] : 0u;
     tot += (wv[u] & 0xFFFFu) + (wv[u] >> 16);
   }
   return tot;
@@ -1096,7 +1098,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     for (uint32_t t0 = 0; REGCOLS ? t0 == 0u : t0 < c2; t0 += kSegWords) {
       const uint32_t nseg = REGCOLS ? c2 : (c2 - t0 < kSegWords ? c2 - t0 : kSegWords);
       uint32_t wv[kSegWords];
-      const uint32_t tot = read_segment(C, wbeg + t0, nseg, T2w, wv);
+      const uint32_t tot = read_segment(C, wbeg + t0, nseg, Lo.T2w_pad - 1u, wv);
       if (REGCOLS) run = wave_excl_scan(tot);
       uint32_t kk[kSegCells], nn4[kSegCells];
       float xx[kSegCells];
@@ -1252,7 +1254,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
         for (uint32_t t0 = 0; (REGCOLS ? t0 == 0u : t0 < c2) && !again; t0 += kSegWords) {
           const uint32_t nseg = REGCOLS ? c2 : (c2 - t0 < kSegWords ? c2 - t0 : kSegWords);
           uint32_t wv[kSegWords];
-          const uint32_t tot = read_segment(C, wbeg + t0, nseg, T2w, wv);
+          const uint32_t tot = read_segment(C, wbeg + t0, nseg, Lo.T2w_pad - 1u, wv);
           if (REGCOLS) run = wave_excl_scan(tot);
           uint32_t kk[kSegCells], nn4[kSegCells];
           float xx[kSegCells];
