@@ -996,7 +996,8 @@ __device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout 
 template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS>
 __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
                                  const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
-                                 WaveStats &st, uint32_t *gpairs) {
+                                 WaveStats &st, uint32_t *gpairs, const uint32_t *pf1 = nullptr,
+                                 const uint32_t *pf2 = nullptr) {
   constexpr uint32_t W = NT / 64;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   uint64_t tc0 = cycles();
@@ -1078,11 +1079,19 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
    * one load per 64 rows instead of a global load latency per row */
   uint32_t Kl = 0u;
   float Al = 0.f;
+  uint32_t pfacc = 0u; /* keeps the prefetch loads */
   for (uint32_t i = w, q = 0; i < T1; i += W, ++q) {
     if ((q & 63u) == 0u) {
       const uint32_t ir = i + lane * W;
       Kl = ir < T1 ? J.K[ir] : 0u;
       Al = ir < T1 ? J.A32[ir] : 0.f;
+      if (q == 0u && pf1) {
+        /* the next task's index arrays, one load per 128-byte line, so that
+         * its build reads them from the cache rather than from HBM; issued
+         * after the row constants, so their waits do not include these */
+        for (uint32_t t = tid; t < (J.N1 + 31u) / 32u; t += NT) pfacc += ((const volatile uint32_t *)pf1)[32u * t];
+        for (uint32_t t = tid; t < (J.N2 + 31u) / 32u; t += NT) pfacc += ((const volatile uint32_t *)pf2)[32u * t];
+      }
     }
     add_bins(S, C, pe0, pe1, pb);
     added = i + 1;
@@ -1316,6 +1325,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   st.cyc[4] += cycles() - tc5;
   if (lane == 0) S.redB[2 * W + w] = xb;
   __syncthreads();
+  if (pfacc == 0xFFFFFFFFu && J.N == 0u) S.redS[0] = pfacc; /* never true: N >= 1 */
   /* p == 1 cells matter only when no cell has p < 1 */
   bool need1 = true;
   for (uint32_t v = 0; v < W; ++v) {
@@ -1410,22 +1420,31 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
   __syncthreads();
   uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * J.N2;
   WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
+  /* each task claims the next one at its start, so that the next task's
+   * explicit index arrays can be prefetched while this one runs */
+  if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
+  __syncthreads();
+  uint64_t pp = s_next;
+  __syncthreads();
   for (;;) {
+    if (pp >= n_perm) break;
     if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
     __syncthreads();
-    const uint64_t pp = s_next;
+    const uint64_t pn = s_next;
     __syncthreads();
-    if (pp >= n_perm) break;
     const uint64_t pid = perm_base + pp;
     const uint32_t *i1 = (EXPLICIT && idx1) ? idx1 + pp * J.N1 : nullptr;
     const uint32_t *i2 = (EXPLICIT && idx2) ? idx2 + pp * J.N2 : nullptr;
+    const bool pfok = EXPLICIT && !PLAIN && idx1 && idx2 && pn < n_perm;
     Shuffle s1, s2;
     if (!EXPLICIT) {
       s1 = shuffle_make(seed, job_id, (uint32_t)pid, 1u, J.N1);
       s2 = shuffle_make(seed, job_id, (uint32_t)pid, 2u, J.N2);
     }
-    BestKey b = optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS>(J, Lo, S, i1, i2, s1, s2, err, st, gpairs);
+    BestKey b = optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS>(
+        J, Lo, S, i1, i2, s1, s2, err, st, gpairs, pfok ? idx1 + pn * J.N1 : nullptr, pfok ? idx2 + pn * J.N2 : nullptr);
     if (threadIdx.x == 0) write_record(J, b, permuted, out + pp, err);
+    pp = pn;
   }
   flush_stats(J, st);
 }
