@@ -989,6 +989,15 @@ __device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout 
   return wave_zero(o);
 }
 
+/* The launch's task counter and where a workgroup puts its next task; with
+ * explicit permutations, their index arrays (for the prefetch). */
+struct NextTask {
+  uint32_t *counter;
+  uint32_t *slot;
+  const uint32_t *idx1, *idx2;
+  uint64_t n_perm;
+};
+
 /* One permutation: overlap grid -> best cell.  Returns the best key in
  * thread 0 (others: undefined).  Rows are interleaved over the waves: every
  * wave adds every row's pairs to its own running column counts, and scans
@@ -996,8 +1005,7 @@ __device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout 
 template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS>
 __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
                                  const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
-                                 WaveStats &st, uint32_t *gpairs, const uint32_t *pf1 = nullptr,
-                                 const uint32_t *pf2 = nullptr) {
+                                 WaveStats &st, uint32_t *gpairs, const NextTask *nt = nullptr) {
   constexpr uint32_t W = NT / 64;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   uint64_t tc0 = cycles();
@@ -1036,6 +1044,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     }
     xb = wave_best(xb);
     if (lane == 0) S.redB[2 * W + w] = xb;
+    if (nt && tid == 0) *nt->slot = atomicAdd(nt->counter, 1u);
     __syncthreads();
     BestKey b = {__builtin_inf(), 0u, kNone};
     if (tid == 0)
@@ -1085,12 +1094,22 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       const uint32_t ir = i + lane * W;
       Kl = ir < T1 ? J.K[ir] : 0u;
       Al = ir < T1 ? J.A32[ir] : 0.f;
-      if (q == 0u && pf1) {
-        /* the next task's index arrays, one load per 128-byte line, so that
-         * its build reads them from the cache rather than from HBM; issued
-         * after the row constants, so their waits do not include these */
-        for (uint32_t t = tid; t < (J.N1 + 31u) / 32u; t += NT) pfacc += ((const volatile uint32_t *)pf1)[32u * t];
-        for (uint32_t t = tid; t < (J.N2 + 31u) / 32u; t += NT) pfacc += ((const volatile uint32_t *)pf2)[32u * t];
+      if (q == 0u && nt && w == 0u) {
+        /* wave 0 claims the workgroup's next task here (every workgroup of
+         * the launch has started by now, so none takes a second task while
+         * another still has none) and loads each 128-byte line of its
+         * explicit index arrays once, so that its build reads them from the
+         * cache rather than from HBM.  Issued after the row constants, so
+         * their waits do not include these. */
+        uint32_t pn = 0u;
+        if (lane == 0) pn = atomicAdd(nt->counter, 1u);
+        pn = (uint32_t)__builtin_amdgcn_readfirstlane((int)pn);
+        if (lane == 0) *nt->slot = pn;
+        if (nt->idx1 && nt->idx2 && pn < nt->n_perm) {
+          const volatile uint32_t *p1 = nt->idx1 + (uint64_t)pn * J.N1, *p2 = nt->idx2 + (uint64_t)pn * J.N2;
+          for (uint32_t t = lane; t < (J.N1 + 31u) / 32u; t += 64u) pfacc += p1[32u * t];
+          for (uint32_t t = lane; t < (J.N2 + 31u) / 32u; t += 64u) pfacc += p2[32u * t];
+        }
       }
     }
     add_bins(S, C, pe0, pe1, pb);
@@ -1420,29 +1439,25 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
   __syncthreads();
   uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * J.N2;
   WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
-  /* each task claims the next one at its start, so that the next task's
-   * explicit index arrays can be prefetched while this one runs */
+  /* the first task; each task claims the next one (optimize_perm) */
   if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
   __syncthreads();
   uint64_t pp = s_next;
   __syncthreads();
+  const NextTask nt = {counter, &s_next, EXPLICIT && !PLAIN ? idx1 : nullptr, EXPLICIT && !PLAIN ? idx2 : nullptr,
+                       n_perm};
   for (;;) {
     if (pp >= n_perm) break;
-    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint64_t pn = s_next;
-    __syncthreads();
     const uint64_t pid = perm_base + pp;
     const uint32_t *i1 = (EXPLICIT && idx1) ? idx1 + pp * J.N1 : nullptr;
     const uint32_t *i2 = (EXPLICIT && idx2) ? idx2 + pp * J.N2 : nullptr;
-    const bool pfok = EXPLICIT && !PLAIN && idx1 && idx2 && pn < n_perm;
     Shuffle s1, s2;
     if (!EXPLICIT) {
       s1 = shuffle_make(seed, job_id, (uint32_t)pid, 1u, J.N1);
       s2 = shuffle_make(seed, job_id, (uint32_t)pid, 2u, J.N2);
     }
-    BestKey b = optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS>(
-        J, Lo, S, i1, i2, s1, s2, err, st, gpairs, pfok ? idx1 + pn * J.N1 : nullptr, pfok ? idx2 + pn * J.N2 : nullptr);
+    BestKey b = optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS>(J, Lo, S, i1, i2, s1, s2, err, st, gpairs, &nt);
+    const uint64_t pn = s_next; /* published by optimize_perm's last barrier */
     if (threadIdx.x == 0) write_record(J, b, permuted, out + pp, err);
     pp = pn;
   }
