@@ -1089,27 +1089,19 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   uint32_t Kl = 0u;
   float Al = 0.f;
   uint32_t pfacc = 0u; /* keeps the prefetch loads */
+  uint32_t pnext = kNone; /* wave 0: the next task */
   for (uint32_t i = w, q = 0; i < T1; i += W, ++q) {
     if ((q & 63u) == 0u) {
       const uint32_t ir = i + lane * W;
       Kl = ir < T1 ? J.K[ir] : 0u;
       Al = ir < T1 ? J.A32[ir] : 0.f;
       if (q == 0u && nt && w == 0u) {
-        /* wave 0 claims the workgroup's next task here (every workgroup of
+        /* wave 0 claims the workgroup's next task here: every workgroup of
          * the launch has started by now, so none takes a second task while
-         * another still has none) and loads each 128-byte line of its
-         * explicit index arrays once, so that its build reads them from the
-         * cache rather than from HBM.  Issued after the row constants, so
-         * their waits do not include these. */
-        uint32_t pn = 0u;
-        if (lane == 0) pn = atomicAdd(nt->counter, 1u);
-        pn = (uint32_t)__builtin_amdgcn_readfirstlane((int)pn);
-        if (lane == 0) *nt->slot = pn;
-        if (nt->idx1 && nt->idx2 && pn < nt->n_perm) {
-          const volatile uint32_t *p1 = nt->idx1 + (uint64_t)pn * J.N1, *p2 = nt->idx2 + (uint64_t)pn * J.N2;
-          for (uint32_t t = lane; t < (J.N1 + 31u) / 32u; t += 64u) pfacc += p1[32u * t];
-          for (uint32_t t = lane; t < (J.N2 + 31u) / 32u; t += 64u) pfacc += p2[32u * t];
-        }
+         * another still has none */
+        if (lane == 0) pnext = atomicAdd(nt->counter, 1u);
+        pnext = (uint32_t)__builtin_amdgcn_readfirstlane((int)pnext);
+        if (lane == 0) *nt->slot = pnext;
       }
     }
     add_bins(S, C, pe0, pe1, pb);
@@ -1331,6 +1323,13 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   if (nc) flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
   uint64_t tc4 = cycles();
   st.cyc[3] += tc4 - tc3;
+  if (nt && w == 0u && nt->idx1 && nt->idx2 && pnext < nt->n_perm) {
+    /* the next task's index arrays, late in this task so that they are
+     * still in the cache when its build reads them */
+    const volatile uint32_t *p1 = nt->idx1 + (uint64_t)pnext * J.N1, *p2 = nt->idx2 + (uint64_t)pnext * J.N2;
+    for (uint32_t t = lane; t < (J.N1 + 31u) / 32u; t += 64u) pfacc += p1[32u * t];
+    for (uint32_t t = lane; t < (J.N2 + 31u) / 32u; t += 64u) pfacc += p2[32u * t];
+  }
   /* tier 3: exact FP64 for cells whose bracket reaches the best bound this
    * wave knows of (its own, and what the other waves have shared so far: no
    * barrier, a looser bound only evaluates more cells) */
