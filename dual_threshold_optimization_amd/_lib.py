@@ -106,6 +106,7 @@ SIGNATURES = {
     "dto_device_count": (_int, []),
     "dto_hypergeometric_pvalue": (_int, [_P(_u64), _P(_u64), _P(_u64), _P(_u64), _u64, _P(_dbl)]),
     "dto_fdr": (_dbl, [_u64, _u64, _u64, _u64, _dbl]),
+    "dto_fdr_checked": (_int, [_u64, _u64, _u64, _u64, _dbl, _P(_dbl)]),
     "dto_generate_thresholds": (_u32, [_u32, _P(_u32), _u32]),
     "dto_ln_factorial_table": (_int, [_u64, _P(_dbl)]),
     "dto_job_create": (_int, [_P(_cp), _P(_u32), _u64, _P(_cp), _P(_u32), _u64, _P(_cp), _u64, _u32, _P(_vp)]),
@@ -140,6 +141,8 @@ INTERNAL = {
     "dto_internal_shuffle_device": (_int, [_u64, _u32, _u32, _u32, _u32, _P(_u32)]),
     "dto_internal_stats": (_int, [_vp, _P(_u64), _int]),
     "dto_internal_philox": (_int, [_P(_u32), _P(_u32), _P(_u32)]),
+    "dto_internal_read_ranked_csv": (_int, [_cp, _vp, _u64, _P(_u64), _P(_u32), _u64, _P(_u64)]),
+    "dto_internal_read_feature_lines": (_int, [_cp, _vp, _u64, _P(_u64), _P(_u64)]),
 }
 
 _lib = None
@@ -165,6 +168,27 @@ def check(rc):
     if rc != DTO_OK:
         raise DtoError(rc, lib().dto_last_error().decode(errors="replace"))
     return rc
+
+
+def read_ranked_csv_native(path):
+    """The library's CSV reader alone (test hook): (ids as bytes, ranks)."""
+    L, n, used = lib(), ctypes.c_uint64(), ctypes.c_uint64()
+    check(L.dto_internal_read_ranked_csv(os.fsencode(path), None, 0, ctypes.byref(used), None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(max(used.value, 1))
+    ranks = np.zeros(max(n.value, 1), dtype=np.uint32)
+    check(L.dto_internal_read_ranked_csv(os.fsencode(path), buf, used.value, ctypes.byref(used), u32p(ranks), n.value,
+                                         ctypes.byref(n)))
+    ids = buf.raw[:used.value].split(b"\0")[:-1] if used.value else []
+    return ids, ranks[:n.value]
+
+
+def read_feature_lines_native(path):
+    """The library's background reader alone (test hook): ids as bytes."""
+    L, n, used = lib(), ctypes.c_uint64(), ctypes.c_uint64()
+    check(L.dto_internal_read_feature_lines(os.fsencode(path), None, 0, ctypes.byref(used), ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(max(used.value, 1))
+    check(L.dto_internal_read_feature_lines(os.fsencode(path), buf, used.value, ctypes.byref(used), ctypes.byref(n)))
+    return buf.raw[:used.value].split(b"\0")[:-1] if used.value else []
 
 
 def device_count():

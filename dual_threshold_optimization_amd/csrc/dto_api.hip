@@ -320,13 +320,26 @@ int dto_device_count(void) {
 }
 
 double dto_fdr(uint64_t l1, uint64_t l2, uint64_t overlap, uint64_t population, double sensitivity) {
-  /* fdr.rs:29-60 (the sensitivity <= 0 panic is the caller's to avoid) */
+  /* fdr.rs:29-60; where the reference panics (sensitivity <= 0, fdr.rs:36-39)
+   * the result is NaN and dto_last_error() holds the panic message */
+  if (sensitivity <= 0.0) {
+    fail(DTO_ERR_INVALID_ARGUMENT, "Sensitivity must be greater than 0.");
+    return __builtin_nan("");
+  }
   double df = fmax((double)overlap / sensitivity, 0.0);
   double b = fmax((double)l1 - df, 0.0);
   double r = fmax((double)l2 - df, 0.0);
   double num = b * r;
   double den = (double)population * (double)overlap;
   return den > 0.0 ? num / den : 0.0;
+}
+
+int dto_fdr_checked(uint64_t l1, uint64_t l2, uint64_t overlap, uint64_t population, double sensitivity,
+                    double *out) {
+  if (!out) return fail(DTO_ERR_INVALID_ARGUMENT, "null output");
+  if (sensitivity <= 0.0) return fail(DTO_ERR_INVALID_ARGUMENT, "Sensitivity must be greater than 0.");
+  *out = dto_fdr(l1, l2, overlap, population, sensitivity);
+  return DTO_OK;
 }
 
 uint32_t dto_generate_thresholds(uint32_t max_rank, uint32_t *out, uint32_t cap) {

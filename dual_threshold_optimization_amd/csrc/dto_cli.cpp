@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/dto.h"
+#include "dto_text.h"
 
 static void usage(FILE *f) {
   fprintf(f,
@@ -70,11 +71,9 @@ static uint32_t max_rank(const std::string &path) {
     size_t comma = line.find(',');
     if (comma != std::string::npos) {
       std::string r = line.substr(comma + 1);
-      size_t a = 0, b = r.size();
-      while (a < b && strchr(" \t\r\n\v\f", r[a])) ++a;
-      while (b > a && strchr(" \t\r\n\v\f", r[b - 1])) --b;
+      if (!r.empty() && r.back() == '\r') r.pop_back();
       uint64_t v = 0;
-      if (parse_u64(r.substr(a, b - a), &v) && (uint32_t)v > mx) mx = (uint32_t)v;
+      if (dto::utf8_valid(r) && parse_u64(dto::rust_trim(r), &v) && (uint32_t)v > mx) mx = (uint32_t)v;
     }
     line.clear();
   };

@@ -8,6 +8,7 @@
  * round exactly as the reference's (Rust never fuses multiply-add).
  */
 #include "dto_host.hpp"
+#include "dto_text.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -298,13 +299,11 @@ int HostJob::derive() {
 /* ------------------------------------------------------------------ */
 /* readers                                                               */
 /* ------------------------------------------------------------------ */
-static bool is_ws(char ch) { return ch == ' ' || ch == '\t' || ch == '\n' || ch == '\r' || ch == '\v' || ch == '\f'; }
-static std::string trim(const std::string &s) {
-  size_t a = 0, b = s.size();
-  while (a < b && is_ws(s[a])) ++a;
-  while (b > a && is_ws(s[b - 1])) --b;
-  return s.substr(a, b - a);
-}
+/* BufRead::lines() hands out each line as a String and fails on a line that
+ * is not valid UTF-8; the readers' .expect("Could not read line") panics
+ * with the io::Error's Debug form. */
+static const char *kBadUtf8 =
+    "Could not read line: Error { kind: InvalidData, message: \"stream did not contain valid UTF-8\" }";
 
 /* BufRead::lines(): split on '\n', drop one trailing '\r', no final empty line. */
 static bool read_lines(const char *path, std::vector<std::string> &lines) {
@@ -330,11 +329,12 @@ int read_ranked_csv(const char *path, std::vector<std::string> &ids, std::vector
   if (!path || !read_lines(path, lines)) return fail(DTO_ERR_IO, std::string("Could not open file: ") + (path ? path : ""));
   for (size_t i = 0; i < lines.size(); ++i) {
     const std::string &line = lines[i];
+    if (!utf8_valid(line)) return fail(DTO_ERR_PARSE, kBadUtf8);
     size_t c = line.find(',');
     if (c == std::string::npos || line.find(',', c + 1) != std::string::npos)
       return fail(DTO_ERR_PARSE, std::string("Invalid format in file ") + path + " at line " + std::to_string(i + 1));
-    std::string feat = trim(line.substr(0, c));
-    std::string rs = trim(line.substr(c + 1));
+    std::string feat = rust_trim(line.substr(0, c));
+    std::string rs = rust_trim(line.substr(c + 1));
     /* usize::from_str: optional '+', then one or more decimal digits, no overflow */
     size_t s0 = (!rs.empty() && rs[0] == '+') ? 1 : 0;
     if (s0 == rs.size()) return fail(DTO_ERR_PARSE, "Invalid rank value");
@@ -354,7 +354,10 @@ int read_ranked_csv(const char *path, std::vector<std::string> &ids, std::vector
 int read_feature_lines(const char *path, std::vector<std::string> &ids) {
   std::vector<std::string> lines;
   if (!path || !read_lines(path, lines)) return fail(DTO_ERR_IO, std::string("Could not open file: ") + (path ? path : ""));
-  for (auto &l : lines) ids.push_back(trim(l));
+  for (auto &l : lines) {
+    if (!utf8_valid(l)) return fail(DTO_ERR_PARSE, kBadUtf8);
+    ids.push_back(rust_trim(l));
+  }
   return DTO_OK;
 }
 
@@ -417,3 +420,39 @@ std::string result_json(const dto_result &r) {
 }
 
 }  // namespace dto
+
+/* Test hooks (not in dto.h): the readers alone, without building a job.
+ * ids are written NUL-terminated one after another into ids_buf. */
+static int pack_ids(const std::vector<std::string> &ids, char *buf, uint64_t cap, uint64_t *used) {
+  uint64_t need = 0;
+  for (auto &s : ids) need += s.size() + 1;
+  if (used) *used = need;
+  if (!buf) return DTO_OK;
+  if (need > cap) return dto::fail(DTO_ERR_INVALID_ARGUMENT, "id buffer too small");
+  for (auto &s : ids) {
+    memcpy(buf, s.c_str(), s.size() + 1);
+    buf += s.size() + 1;
+  }
+  return DTO_OK;
+}
+
+extern "C" int dto_internal_read_ranked_csv(const char *path, char *ids_buf, uint64_t ids_cap, uint64_t *ids_used,
+                                            uint32_t *ranks, uint64_t cap, uint64_t *n) {
+  std::vector<std::string> ids;
+  std::vector<uint32_t> r;
+  int rc = dto::read_ranked_csv(path, ids, r);
+  if (rc) return rc;
+  if (n) *n = ids.size();
+  if (ranks)
+    for (uint64_t i = 0; i < cap && i < r.size(); ++i) ranks[i] = r[i];
+  return pack_ids(ids, ids_buf, ids_cap, ids_used);
+}
+
+extern "C" int dto_internal_read_feature_lines(const char *path, char *ids_buf, uint64_t ids_cap, uint64_t *ids_used,
+                                               uint64_t *n) {
+  std::vector<std::string> ids;
+  int rc = dto::read_feature_lines(path, ids);
+  if (rc) return rc;
+  if (n) *n = ids.size();
+  return pack_ids(ids, ids_buf, ids_cap, ids_used);
+}

@@ -35,10 +35,13 @@ def intersect_genes(genes1, genes2):
 
 def fdr(ranked_list_1_len, ranked_list_2_len, overlap_len, population_size, sensitivity=0.8):
     """src/stat_operations/fdr.rs:29-60."""
-    if sensitivity <= 0.0:
-        raise ValueError("Sensitivity must be greater than 0.")
-    return _lib.lib().dto_fdr(int(ranked_list_1_len), int(ranked_list_2_len), int(overlap_len),
-                              int(population_size), float(sensitivity))
+    out = ctypes.c_double()
+    rc = _lib.lib().dto_fdr_checked(int(ranked_list_1_len), int(ranked_list_2_len), int(overlap_len),
+                                    int(population_size), float(sensitivity), ctypes.byref(out))
+    if rc == _lib.DTO_ERR_INVALID_ARGUMENT:
+        raise ValueError(_lib.lib().dto_last_error().decode())
+    _lib.check(rc)
+    return out.value
 
 
 def _to_c_record(r):

@@ -94,9 +94,14 @@ int dto_device_count(void);
  * for `count` quadruples, evaluated on the GPU. */
 int dto_hypergeometric_pvalue(const uint64_t *N, const uint64_t *K, const uint64_t *n,
                               const uint64_t *k, uint64_t count, double *out);
-/* fdr (src/stat_operations/fdr.rs:29-60); closed form, host. */
+/* fdr (src/stat_operations/fdr.rs:29-60); closed form, host.  Where the
+ * reference panics (sensitivity <= 0, fdr.rs:36-39) it returns NaN and sets
+ * dto_last_error(); dto_fdr_checked reports that case as
+ * DTO_ERR_INVALID_ARGUMENT instead. */
 double dto_fdr(uint64_t set1_len, uint64_t set2_len, uint64_t overlap, uint64_t population,
                double sensitivity);
+int dto_fdr_checked(uint64_t set1_len, uint64_t set2_len, uint64_t overlap, uint64_t population,
+                    double sensitivity, double *out);
 /* generate_thresholds (src/collections/ranked.rs:359-375): writes up to cap
  * values into out (may be NULL) and returns how many the reference makes. */
 uint32_t dto_generate_thresholds(uint32_t max_rank, uint32_t *out, uint32_t cap);

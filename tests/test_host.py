@@ -395,3 +395,75 @@ def test_cli_without_device_fails_loudly(dto):
     g = os.path.join(ROOT, "tests", "golden")
     r = _cli(dto, "-1", os.path.join(g, "ranklist1.csv"), "-2", os.path.join(g, "ranklist2.csv"), "-p", "5")
     assert r.returncode == 101 and "no HIP device" in r.stderr and r.stdout == ""
+
+
+def _write_bytes(tmp_path, name, data):
+    p = tmp_path / name
+    p.write_bytes(data)
+    return str(p)
+
+
+def test_readers_unicode_trim_and_invalid_utf8(dto, tmp_path):
+    """str::trim strips Unicode White_Space (read_ranked_feature_list_from_csv.rs:62-63,
+    read_feature_list_from_file.rs:50); a line that is not valid UTF-8 makes
+    lines() fail and the reader panic (read_ranked_feature_list_from_csv.rs:57).
+    The library's readers and the Python mirror agree with both rules."""
+    from dual_threshold_optimization_amd import _lib
+    from dual_threshold_optimization_amd.read import read_feature_list_from_file, read_ranked_feature_list_from_csv
+    ws = "  　\u0085    \t\x0b"
+    text = ("%sgA%s,%s3%s\r\n" % (ws, ws, ws, ws)) + "gB​,+1\n" + "\x1cgC\x1f,2\n"
+    p = _write_bytes(tmp_path, "u.csv", text.encode())
+    ids, ranks = _lib.read_ranked_csv_native(p)
+    # U+200B (zero width space) and U+001C/U+001F are not White_Space: kept
+    assert ids == ["gA".encode(), "gB​".encode(), "\x1cgC\x1f".encode()]
+    assert list(ranks) == [3, 1, 2]
+    rl = read_ranked_feature_list_from_csv(p)
+    assert rl.ids() == ["gB​", "\x1cgC\x1f", "gA"] and list(rl.ranks()) == [1, 2, 3]
+    bgp = _write_bytes(tmp_path, "bg.txt", ("%sx%s\n \ny\x1c\n" % (ws, ws)).encode())
+    assert _lib.read_feature_lines_native(bgp) == [b"x", b"", "y\x1c".encode()]
+    assert read_feature_list_from_file(bgp).ids() == ["x", "", "y\x1c"]
+    bad_lines = [b"a\xff,1\n", b"a\xc0\x80,1\n", b"a\xed\xa0\x80,1\n", b"a\xf4\x90\x80\x80,1\n", b"a\xe2\x82,1\n"]
+    for k, bad in enumerate(bad_lines):
+        p = _write_bytes(tmp_path, "bad%d.csv" % k, b"ok,1\n" + bad)
+        with pytest.raises(_lib.DtoError) as e:
+            _lib.read_ranked_csv_native(p)
+        assert e.value.code == _lib.DTO_ERR_PARSE and "valid UTF-8" in e.value.message
+        with pytest.raises(ValueError, match="valid UTF-8"):
+            read_ranked_feature_list_from_csv(p)
+        pb = _write_bytes(tmp_path, "badbg%d.txt" % k, b"ok\n" + bad)
+        with pytest.raises(_lib.DtoError) as e:
+            _lib.read_feature_lines_native(pb)
+        assert e.value.code == _lib.DTO_ERR_PARSE
+        with pytest.raises(ValueError, match="valid UTF-8"):
+            read_feature_list_from_file(pb)
+    # well-formed 2-, 3- and 4-byte sequences are accepted
+    p = _write_bytes(tmp_path, "ok.csv", "é,1\n€,2\n\U0001f600,3\n".encode())
+    assert _lib.read_ranked_csv_native(p)[0] == ["é".encode(), "€".encode(), "\U0001f600".encode()]
+    # lines are read in order: a format error before the bad line wins, and after it loses
+    p = _write_bytes(tmp_path, "o1.csv", b"a,1,2\nb\xff,1\n")
+    with pytest.raises(_lib.DtoError, match="Invalid format"):
+        _lib.read_ranked_csv_native(p)
+    with pytest.raises(ValueError, match="Invalid format"):
+        read_ranked_feature_list_from_csv(p)
+    p = _write_bytes(tmp_path, "o2.csv", b"b\xff,1\na,1,2\n")
+    with pytest.raises(_lib.DtoError, match="valid UTF-8"):
+        _lib.read_ranked_csv_native(p)
+    with pytest.raises(ValueError, match="valid UTF-8"):
+        read_ranked_feature_list_from_csv(p)
+
+
+def test_fdr_sensitivity_error_path(dto):
+    """fdr.rs:36-39 panics for sensitivity <= 0: the ABI reports it as an error."""
+    from dual_threshold_optimization_amd import _lib
+    from dual_threshold_optimization_amd.stat_operations import fdr
+    import ctypes
+    import math
+    out = ctypes.c_double()
+    for s in (0.0, -0.5, -0.0):
+        assert _lib.lib().dto_fdr_checked(200, 150, 50, 1000, s, ctypes.byref(out)) == _lib.DTO_ERR_INVALID_ARGUMENT
+        assert _lib.lib().dto_last_error() == b"Sensitivity must be greater than 0."
+        assert math.isnan(_lib.lib().dto_fdr(200, 150, 50, 1000, s))
+        with pytest.raises(ValueError, match="Sensitivity must be greater than 0."):
+            fdr(200, 150, 50, 1000, s)
+    assert _lib.lib().dto_fdr_checked(200, 150, 50, 1000, 0.8, ctypes.byref(out)) == 0
+    assert abs(out.value - 0.240625) < 1e-4 and out.value == fdr(200, 150, 50, 1000)

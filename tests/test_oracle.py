@@ -115,16 +115,20 @@ def test_thresholds_properties(O):
 
 # ---- A6 / A7 / A10: the grid and its optimum ------------------------------------
 
-def brute_grid(ids1, r1, ids2, r2, N, idx1=None, idx2=None):
-    """Per-cell set intersection over stably sorted lists (exact rationals)."""
+def brute_grid(ids1, r1, ids2, r2, N, idx1=None, idx2=None, T1=None, T2=None):
+    """Per-cell set intersection over stably sorted lists (exact rationals).
+    T1/T2: the threshold values to evaluate (default: every integer from 1 to
+    the list's largest rank); a set holds the features of rank <= threshold."""
     o1 = sorted(range(len(r1)), key=lambda i: r1[i])
     o2 = sorted(range(len(r2)), key=lambda i: r2[i])
     g1, s1 = [ids1[i] for i in o1], [int(r1[i]) for i in o1]
     g2, s2 = [ids2[i] for i in o2], [int(r2[i]) for i in o2]
     p1 = g1 if idx1 is None else [g1[int(i)] for i in idx1]
     p2 = g2 if idx2 is None else [g2[int(i)] for i in idx2]
-    T1 = [t for t in range(1, s1[-1] + 1)]
-    T2 = [t for t in range(1, s2[-1] + 1)]
+    if T1 is None:
+        T1 = [t for t in range(1, s1[-1] + 1)]
+    if T2 is None:
+        T2 = [t for t in range(1, s2[-1] + 1)]
     out = []
     for a in T1:
         A = [p1[q] for q in range(len(p1)) if s1[q] <= a]
@@ -185,6 +189,51 @@ def test_grid_vs_brute_force(O, seed):
     best = O.optimize(l1, l2, N, idx1, idx2)
     key = min((r.pvalue, -r.intersection_size, r.rank1, r.rank2) for r in recs)
     assert (best.pvalue, -best.intersection_size, best.rank1, best.rank2) == key
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_custom_thresholds_vs_brute_force(O, seed):
+    """or_list_set_thresholds (custom threshold lists, which replace
+    generate_thresholds in ranked.rs:359-375 and feed the set rule of
+    ranked.rs:561-568 / permuted.rs:90-101): the records on explicit threshold
+    lists equal per-cell set intersection with sets {rank <= t}.  The lists
+    cover thresholds between ranks, equal to a tied rank, below the smallest
+    rank (an empty set), above the largest rank, and a single threshold."""
+    rng = np.random.default_rng(200 + seed)
+    n = int(rng.integers(3, 36))
+    i1, r1, i2, r2 = random_lists(rng, n, tie_frac=0.3, dup=seed % 3 == 2, max_rank=int(rng.integers(n, 3 * n)))
+    l1, l2 = O.OracleList(i1, r1), O.OracleList(i2, r2)
+    m1, m2 = int(max(r1)), int(max(r2))
+
+    def pick(m, case):
+        if case == 0:
+            return [m + 5]                                   # one threshold, above every rank
+        if case == 1:
+            return [int(min(r1 if m == m1 else r2))]         # one threshold, the smallest rank
+        vals = set(int(x) for x in rng.choice(np.arange(0, 3 * m + 2), size=min(3 * m, 12), replace=False))
+        vals |= {int(x) for x in rng.choice(r1 if m == m1 else r2, size=3)}  # tied / exact ranks
+        vals.add(m + 1 + int(rng.integers(0, 5)))              # past the largest rank
+        return sorted(vals)
+    t1 = np.array(pick(m1, seed % 4), dtype=np.uint32)
+    t2 = np.array(pick(m2, (seed + 1) % 4), dtype=np.uint32)
+    l1.set_thresholds(t1)
+    l2.set_thresholds(t2)
+    assert l1.thresholds == list(t1) and l2.thresholds == list(t2)
+    N = 2 * n + 10
+    for idx1, idx2 in ((None, None), (rng.permutation(len(i1)), rng.permutation(len(i2)))):
+        recs = O.process_threshold_pairs(l1, l2, N, idx1, idx2)
+        brute = brute_grid(i1, r1, i2, r2, N, idx1, idx2, T1=list(t1), T2=list(t2))
+        assert len(recs) == len(brute) == len(t1) * len(t2)
+        for r, (a, b, A, B, k) in zip(recs, brute):
+            assert (r.rank1, r.rank2, r.set1_len, r.set2_len, r.intersection_size) == (a, b, A, B, k)
+            assert r.pvalue == O.hypergeometric_pvalue(N, A, B, k)
+        k, p = O.fast_grid(l1, l2, N, idx1, idx2)
+        assert np.array_equal(k.ravel(), [x[4] for x in brute])
+        best = O.optimize(l1, l2, N, idx1, idx2, fast=True)
+        key = min((r.pvalue, -r.intersection_size, r.rank1, r.rank2) for r in recs)
+        assert (best.pvalue, -best.intersection_size, best.rank1, best.rank2) == key
+    with pytest.raises(ValueError):
+        l1.set_thresholds(np.array([3, 3], dtype=np.uint32))
 
 
 def test_duplicate_id_asymmetry(O):
