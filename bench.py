@@ -41,13 +41,15 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--perms", type=int, default=10000)
     p.add_argument("--plain-perms", type=int, default=256)
-    p.add_argument("--dense-perms", type=int, default=2000)
+    p.add_argument("--dense-perms", type=int, default=10000)
     p.add_argument("--dense-plain-perms", type=int, default=64)
     p.add_argument("--batch-jobs", type=int, default=512)
     p.add_argument("--batch-perms", type=int, default=1000)
-    p.add_argument("--serving-requests", type=int, default=400)
-    p.add_argument("--serving-rate", type=float, default=40.0, help="offered load, requests/s")
+    p.add_argument("--serving-requests", type=int, default=5000)
+    p.add_argument("--serving-rate", type=float, default=500.0, help="offered load, requests/s")
+    p.add_argument("--serving-list-pairs", type=int, default=500, help="distinct generated list pairs")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline sample length")
+    p.add_argument("--cli-runs", type=int, default=10, help="timed runs of the CLI on the README lists")
     p.add_argument("--only", choices=["all", "headline", "calibrate-copy"], default="all")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_optimize_genome_20k_pmc.txt"))
@@ -210,25 +212,35 @@ def main():
         dj.run_device(DP, dout.data_ptr(), seed=4, stream=sptr)
         torch.cuda.synchronize()
         dense_rate = DP / (time.perf_counter() - t)
-        rs = np.frombuffer(dout.cpu().numpy().tobytes(), dtype=dto._lib.RECORD_DTYPE).copy()
         DS = min(args.dense_plain_perms, DP)
         t = time.perf_counter()
         dj.run_device(DS, dout.data_ptr(), seed=4, mode="plain", stream=sptr)
         torch.cuda.synchronize()
         dense_plain_rate = DS / (time.perf_counter() - t)
-        rp = np.frombuffer(dout[:DS * RECORD_BYTES].cpu().numpy().tobytes(), dtype=dto._lib.RECORD_DTYPE)
-        ps, pp = rs["pvalue"][:DS], rp["pvalue"]
-        rel = np.where(pp > 0, np.abs(ps - pp) / np.where(pp > 0, pp, 1.0), np.abs(ps - pp))
-        agree = float(np.mean((rs["idx1"][:DS] == rp["idx1"]) & (rs["idx2"][:DS] == rp["idx2"])))
-        log("dense 2k x 2k: screened %.0f perms/s, plain FP64 %.1f perms/s; max rel err of min p %.3g, "
-            "argmin agreement %.4f on %d permutations" % (dense_rate, dense_plain_rate, float(rel.max()), agree, DS))
+        # accuracy against the all-FP64 oracle: its records for the unpermuted
+        # grid and the explicit permutations of tests/golden/dense_2k_genome_20k.json
+        with open(os.path.join(ROOT, "tests", "golden", "dense_2k_genome_20k.json")) as f:
+            fx = json.load(f)
+        FP = len(fx["permuted"])
+        e1, e2 = W.explicit_permutations(FP, dj.len1, dj.len2, 21)
+        res, perm = dj.run(FP, idx1=e1, idx2=e2, mode="screened")
+        got = [(int(res.unpermuted.rank1), int(res.unpermuted.rank2), float(res.unpermuted.pvalue))]
+        got += [(int(r["rank1"]), int(r["rank2"]), float(r["pvalue"])) for r in perm]
+        want = [(d["rank1"], d["rank2"], float.fromhex(d["pvalue"])) for d in [fx["unpermuted"]] + fx["permuted"]]
+        rel = [abs(g[2] - w[2]) / w[2] if w[2] > 0 else abs(g[2] - w[2]) for g, w in zip(got, want)]
+        agree = float(np.mean([g[:2] == w[:2] for g, w in zip(got, want)]))
+        log("dense 2k x 2k: screened %.0f perms/s (Philox), plain FP64 %.1f perms/s; vs the FP64 oracle on %d grids: "
+            "max rel err of min p %.3g, argmin agreement %.4f" % (dense_rate, dense_plain_rate, len(want),
+                                                                   max(rel), agree))
         result["dense_2k_thresholds"] = {
             "workload": "genome_20k job with thresholds 10, 20, ..., 20,000 on both lists (4,000,000 cells), "
-                        "Philox permutations",
+                        "%d Philox permutations per launch" % DP,
             "screened_perms_per_s": round(dense_rate, 2), "plain_fp64_perms_per_s": round(dense_plain_rate, 3),
             "screened_vs_plain": round(dense_rate / dense_plain_rate, 2),
-            "max_rel_err_min_p_vs_fp64": float(rel.max()), "argmin_agreement_vs_fp64": agree,
-            "compared_permutations": DS}
+            "max_rel_err_min_p_vs_fp64_oracle": float(max(rel)), "argmin_agreement_vs_fp64_oracle": agree,
+            "oracle_grids_compared": len(want),
+            "oracle": "committed all-FP64 oracle records (tests/golden/make_dense_fixture.py): the unpermuted grid "
+                      "and %d explicit permutations" % FP}
         del dout
         dj.close()
 
@@ -277,43 +289,163 @@ def main():
         b.close()
         del jobs
 
-    # ---- serving: Poisson arrivals, continuous batching ----
+    # ---- serving: Poisson arrivals, continuous batching vs one launch per job ----
     if args.serving_requests > 0:
-        R = args.serving_requests
-        arr, Ns, Ps, planted, bgf, seeds = W.serving_requests(n_req=R, seed=5)
-        arr = arr / args.serving_rate
-        t = time.perf_counter()
-        lists = [W.serving_job_lists(int(Ns[i]), int(planted[i]), float(bgf[i]), int(seeds[i])) for i in range(R)]
-        log("serving: %d requests generated (%.1fs)" % (R, time.perf_counter() - t))
-        sv = dto.Server(perms_per_slice=256, seed=5)
-        lat, done, i = [], 0, 0
-        t0 = time.perf_counter()
-        while done < R:
-            now = time.perf_counter() - t0
-            while i < R and arr[i] <= now:
-                a, b_, c, d, bgl = lists[i]
-                jb = dto.Job.from_lists(a, b_, c, d, background=bgl, drop_outside_background=True)
-                sv.submit(jb, int(Ps[i]))
-                i += 1
-            for rid, res, l in sv.poll(timeout_ms=1):
-                lat.append(l)
-                done += 1
-            if i < R and done < R:
-                time.sleep(max(0.0, min(0.001, arr[i] - (time.perf_counter() - t0))))
-        span = time.perf_counter() - t0
-        sv.close()
-        lat = np.array(lat)
-        result["serving"] = {"requests": R, "offered_rate_per_s": args.serving_rate,
-                             "p50_latency_s": round(float(np.percentile(lat, 50)), 5),
-                             "p99_latency_s": round(float(np.percentile(lat, 99)), 5),
-                             "perms_per_s": round(float(np.sum(Ps[:R])) / span, 2), "span_s": round(span, 3)}
-        log("serving: p50 %.4fs p99 %.4fs" % (result["serving"]["p50_latency_s"], result["serving"]["p99_latency_s"]))
+        result["serving"] = serving(dto, W, np, args)
+
+    # ---- the reference's own published run: README lists, -p 1000, CLI wall time ----
+    if args.cli_runs > 0:
+        result["cli_readme_p1000"] = cli_wall(args.cli_runs)
 
     # ---- CPU baseline: the reference algorithm restated, bounded sample ----
     if not args.no_cpu:
         result["cpu_baseline"], result["cpu_optimized"] = cpu_baseline(ids1, r1, ids2, r2, bg, args.cpu_seconds)
     print(json.dumps(result))
     return 0
+
+
+def _make_jobs(dto, sets, which, threads):
+    """Deferred jobs (tables built on the host, uploaded when first needed)
+    for requests whose list pairs are sets[which[i]]; built on a thread pool
+    (the library call releases the GIL; the id arrays are built once per
+    list pair)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(w):
+        c1, r1, c2, r2, cb, nb = sets[w]
+        return dto.Job.from_c_arrays(c1, r1, c2, r2, cb, nb, drop_outside_background=True, defer_upload=True)
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(one, which))
+
+
+def serving(dto, W, np, args):
+    """The serving workload: R requests with Poisson arrivals at the offered
+    rate, N log-uniform in [1,000, 30,000], background 80-100% of the union,
+    P in {100, 1,000, 10,000} w.p. {.5, .4, .1}, planted 0-5% of N.  The same
+    arrivals go (a) to the continuous-batching server and (b) to the plain
+    path, one job per launch in arrival order (Job.run: upload, unpermuted
+    task, P permutations, results); latency = completion - arrival."""
+    R = args.serving_requests
+    arr, Ns, Ps, planted, bgf, seeds = W.serving_requests(n_req=R, seed=5)
+    arr = arr / args.serving_rate
+    threads, _ = host_threads()
+    # the first G requests' list pairs are generated; request i uses pair i % G
+    # (its own job, request id and Philox permutations)
+    G = min(R, args.serving_list_pairs)
+    t = time.perf_counter()
+    sets = []
+    for g in range(G):
+        a, b, c, d, bg = W.serving_job_lists(int(Ns[g]), int(planted[g]), float(bgf[g]), int(seeds[g]))
+        sets.append((dto._lib.cstr_array(a), np.ascontiguousarray(b, dtype=np.uint32), dto._lib.cstr_array(c),
+                     np.ascontiguousarray(d, dtype=np.uint32), dto._lib.cstr_array(bg), len(bg)))
+    which = [i % G for i in range(R)]
+    Nreq = np.array([Ns[w] for w in which])
+    log("serving: %d list pairs generated for %d requests (%.1fs)" % (G, R, time.perf_counter() - t))
+    out = {"requests": R, "distinct_list_pairs": G, "offered_rate_per_s": args.serving_rate,
+           "total_permutations": int(np.sum(Ps[:R])), "mean_N": round(float(np.mean(Nreq)), 1),
+           "workload": "Poisson arrivals; N log-uniform [1k, 30k]; background 80-100%; P in {100, 1k, 10k} "
+                       "w.p. {.5, .4, .1}; planted 0-5%"}
+
+    # (a) continuous batching
+    t = time.perf_counter()
+    jobs = _make_jobs(dto, sets, which, threads)
+    log("serving: %d deferred jobs built (%.1fs)" % (R, time.perf_counter() - t))
+    sv = dto.Server(hbm_budget_bytes=8 << 30, perms_per_slice=256, seed=5)
+    lat, done, i = [], 0, 0
+    t0 = time.perf_counter()
+    while done < R:
+        now = time.perf_counter() - t0
+        while i < R and arr[i] <= now:
+            sv.submit(jobs[i], int(Ps[i]))
+            jobs[i] = None
+            i += 1
+        for rid, res, l in sv.poll(timeout_ms=1):
+            lat.append(l)
+            done += 1
+        if i < R and done < R:
+            time.sleep(max(0.0, min(0.001, arr[i] - (time.perf_counter() - t0))))
+    span = time.perf_counter() - t0
+    sv.close()
+    lat = np.array(lat)
+    out.update({"p50_latency_s": round(float(np.percentile(lat, 50)), 5),
+                "p99_latency_s": round(float(np.percentile(lat, 99)), 5),
+                "perms_per_s": round(float(np.sum(Ps[:R])) / span, 2), "span_s": round(span, 3)})
+    log("serving (continuous batching): p50 %.4fs p99 %.4fs, %.0f perms/s over %.2fs"
+        % (out["p50_latency_s"], out["p99_latency_s"], out["perms_per_s"], span))
+
+    # (b) one job per launch, same arrivals, first come first served
+    jobs = _make_jobs(dto, sets, which, threads)
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(R):
+        now = time.perf_counter() - t0
+        if arr[i] > now:
+            time.sleep(arr[i] - now)
+        jobs[i].run(int(Ps[i]), seed=5 + i)
+        lat.append(time.perf_counter() - t0 - arr[i])
+        jobs[i].close()
+        jobs[i] = None
+    span = time.perf_counter() - t0
+    lat = np.array(lat)
+    out["one_launch_per_job"] = {"p50_latency_s": round(float(np.percentile(lat, 50)), 5),
+                                 "p99_latency_s": round(float(np.percentile(lat, 99)), 5),
+                                 "perms_per_s": round(float(np.sum(Ps[:R])) / span, 2), "span_s": round(span, 3)}
+    log("serving (one job per launch): p50 %.4fs p99 %.4fs, %.0f perms/s over %.2fs"
+        % (out["one_launch_per_job"]["p50_latency_s"], out["one_launch_per_job"]["p99_latency_s"],
+           out["one_launch_per_job"]["perms_per_s"], span))
+    return out
+
+
+def cli_wall(runs):
+    """Wall time of the drop-in CLI (a fresh process per run: HIP start-up,
+    reading the files, the unpermuted task and 1,000 Philox permutations on the
+    GPU, JSON out) on the reference's README lists, the run the reference
+    times with hyperfine (profiling/hyperfine_results.json:4-11: mean 0.0705 s,
+    median 0.0685 s, min 0.0671 s at --threads 28 on an unstated CPU)."""
+    import subprocess
+    import numpy as np
+    exe = os.path.join(ROOT, "dual_threshold_optimization_amd", "dto_amd")
+    g = os.path.join(ROOT, "tests", "golden")
+    cmd = [exe, "-1", os.path.join(g, "ranklist1.csv"), "-2", os.path.join(g, "ranklist2.csv"), "-p", "1000",
+           "-t", "1", "--seed", "1"]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=120)  # warm-up (page cache, code objects)
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+        ts.append(time.perf_counter() - t)
+    ts = np.array(ts)
+    out = {"runs": runs, "mean_s": round(float(ts.mean()), 4), "median_s": round(float(np.median(ts)), 4),
+           "min_s": round(float(ts.min()), 4),
+           "command": "dto_amd -1 tests/golden/ranklist1.csv -2 tests/golden/ranklist2.csv -p 1000 -t 1 --seed 1",
+           "reference_hyperfine": {"mean_s": 0.0705, "median_s": 0.0685, "min_s": 0.0671, "threads": 28,
+                                   "hardware": "unstated CPU (context, not a target)",
+                                   "source": "profiling/hyperfine_results.json:4-11"}}
+    log("CLI README run, -p 1000: mean %.4fs median %.4fs (reference hyperfine mean 0.0705s, 28 threads)"
+        % (out["mean_s"], out["median_s"]))
+    return out
+
+
+def host_threads():
+    """Threads for the CPU baseline: every CPU this process may run on, up to
+    the 16 host CPUs a one-GPU box allots per GPU, and what the host is."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = max(1, min(16, allowed))
+    return threads, {"nproc": os.cpu_count(), "affinity_cpus": allowed, "cpu_model": model,
+                     "threads_used": threads,
+                     "note": "threads = the CPUs this process may use, capped at the 16 host CPUs allotted per GPU"}
 
 
 def cpu_baseline(ids1, r1, ids2, r2, bg, target_s):
@@ -329,7 +461,7 @@ def cpu_baseline(ids1, r1, ids2, r2, bg, target_s):
     f2 = [(a, b) for a, b in zip(ids2, r2) if a in keep]
     l1 = O.OracleList([a for a, _ in f1], [b for _, b in f1])
     l2 = O.OracleList([a for a, _ in f2], [b for _, b in f2])
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_threads()
     rng = np.random.default_rng(7)
     a = np.stack([rng.permutation(len(f1)) for _ in range(threads)])
     b = np.stack([rng.permutation(len(f2)) for _ in range(threads)])
@@ -344,7 +476,7 @@ def cpu_baseline(ids1, r1, ids2, r2, bg, target_s):
     dt = time.perf_counter() - t
     rate = cells / dt / cells_per_perm
     log("cpu reference-algorithm port: %.4f perms/s on %d threads (stride %d, %.1fs)" % (rate, threads, stride, dt))
-    base = {"value": round(rate, 6), "unit": "permutations/s", "cores": threads, "kind": "port",
+    base = {"value": round(rate, 6), "unit": "permutations/s", "cores": threads, "kind": "port", "host": host,
             "sample": "genome_20k, %d permutations (one per thread), every %d-th of %d threshold1 rows x all %d "
                       "threshold2 columns (%d cells, %.1fs); rate = cells/s / cells per permutation"
                       % (threads, stride, len(l1.thresholds), len(l2.thresholds), cells, dt)}

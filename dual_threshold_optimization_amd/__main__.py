@@ -5,7 +5,7 @@ import os
 import sys
 
 from . import _lib
-from .dto import Job
+from .dto import Job, tie_warnings
 from .read import read_ranked_feature_list_from_csv
 from .stat_operations import result_json
 
@@ -49,7 +49,13 @@ def main(argv=None):
         if a.multi_node:
             raise _lib.DtoError(1, "Multi-node mode requires MPI. Rebuild with the 'mpi' feature enabled.")
         seed = a.seed if a.seed is not None else int.from_bytes(os.urandom(8), "little")
-        res, _ = job.run(a.permutations, seed=seed, mode=a.mode)
+        res, perm = job.run(a.permutations, seed=seed, mode=a.mode)
+        # each task's optimize warns about its own ties (optimize_main.rs:86-108)
+        for line in tie_warnings(res.unpermuted):
+            print(line, file=err)
+        for r in perm:
+            for line in tie_warnings(r):
+                print(line, file=err)
     except _lib.DtoError as e:
         print("thread 'main' panicked:\n%s" % e.message, file=err)
         return 101

@@ -27,6 +27,7 @@ DTO_ERR_PARSE = 10
 DTO_ERR_BUSY = 11
 
 DTO_DROP_OUTSIDE_BACKGROUND = 0x1
+DTO_DEFER_UPLOAD = 0x2
 DTO_MODE_SCREENED = 0
 DTO_MODE_PLAIN = 1
 MODES = {"screened": DTO_MODE_SCREENED, "plain": DTO_MODE_PLAIN}
@@ -52,14 +53,14 @@ class Record(ctypes.Structure):
         ("permuted", ctypes.c_int32),
         ("idx1", ctypes.c_uint32),
         ("idx2", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("ties", ctypes.c_uint32),
     ]
 
 
 RECORD_DTYPE = np.dtype([
     ("rank1", "<u8"), ("rank2", "<u8"), ("set1_len", "<u8"), ("set2_len", "<u8"),
     ("population_size", "<u8"), ("intersection_size", "<u8"), ("pvalue", "<f8"),
-    ("permuted", "<i4"), ("idx1", "<u4"), ("idx2", "<u4"), ("reserved", "<u4"),
+    ("permuted", "<i4"), ("idx1", "<u4"), ("idx2", "<u4"), ("ties", "<u4"),
 ])
 assert RECORD_DTYPE.itemsize == ctypes.sizeof(Record) == 72
 

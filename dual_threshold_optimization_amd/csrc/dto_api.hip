@@ -67,7 +67,7 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
   L.T2w_pad = (T2w + 4u) & ~3u; /* at least one zero padding word */
   /* the screen table (or, without it, the pair staging) sits at offset 0 so
    * that table reads address it with the scaled index alone */
-  const uint32_t hb = with_h ? 4u * (uint32_t)(h.N + 1) : 0u, pb = 4u * std::max(N2, 1u);
+  const uint32_t hb = with_h ? 4u * (uint32_t)(h.N + 1) : 0u, pb = 2u * std::max(N2, 1u);
   uint32_t fixed = 0;
   auto region = [&](uint32_t &field, uint32_t bytes) {
     field = fixed;
@@ -85,7 +85,7 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
   region(L.cand, 8u * (uint32_t)W * (uint32_t)kCandCap);
   region(L.cand3, 16u * (uint32_t)W * (uint32_t)kCand3Cap);
   region(L.exptab, 8u * 256u);
-  region(L.red, 80u * (uint32_t)W);
+  region(L.red, (16u + 3u * (uint32_t)sizeof(BestKey) + 16u) * (uint32_t)W);
   region(L.misc, 16u);
   uint32_t base = 0;
   L.h = L.pairs = 0;
@@ -120,31 +120,46 @@ static size_t put(std::vector<unsigned char> &buf, const std::vector<T> &v) {
   return off;
 }
 
-int upload_job(dto_job *j) {
+int prepare_job(dto_job *j) {
   int dev;
   int rc = current_device(&dev);
   if (rc) return rc;
   DevProps pr;
   if ((rc = device_props(dev, &pr))) return rc;
   const HostJob &h = j->h;
-  std::vector<unsigned char> buf;
-  size_t o_bin1 = put(buf, h.bin1), o_bin2 = put(buf, h.bin2), o_join = put(buf, h.join);
-  size_t o_joff = put(buf, h.join_off), o_jpos = put(buf, h.join_pos), o_K = put(buf, h.K), o_n = put(buf, h.n);
-  size_t o_t1 = put(buf, h.thr1), o_t2 = put(buf, h.thr2), o_LF = put(buf, h.LF), o_D = put(buf, h.D);
-  size_t o_H = put(buf, h.H32), o_A = put(buf, h.A32), o_B = put(buf, h.B32);
-  size_t o_err = put(buf, std::vector<int>(4, 0));
-  size_t o_ctr = put(buf, std::vector<uint32_t>(kCounterRing, 0));
-  size_t o_stats = put(buf, std::vector<unsigned long long>(16, 0));
-  if (j->arena) {
-    (void)hipFree(j->arena);
-    j->arena = nullptr;
-  }
-  HIP_TRY(hipMalloc(&j->arena, buf.size()));
-  HIP_TRY(hipMemcpy(j->arena, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  /* LDS: keep the screen table resident when it fits */
+  const size_t lim = pr.lds_per_block;
+  LdsLayout with_h = make_layout(h, kNT / 64, true, lim), without_h = make_layout(h, kNT / 64, false, lim);
+  if (without_h.total > lim)
+    return fail(DTO_ERR_TOO_LARGE, "job needs " + std::to_string(without_h.total) + " B of LDS per workgroup, device has " +
+                                       std::to_string(lim));
+  j->lo_plain = without_h;
+  j->lo_screen = with_h.total <= lim ? with_h : without_h;
+  std::vector<unsigned char> &buf = j->image;
+  buf.clear();
+  ArenaOffsets &o = j->off;
+  o.bin1 = put(buf, h.bin1);
+  o.bin2 = put(buf, h.bin2);
+  o.join = put(buf, h.join);
+  o.joff = put(buf, h.join_off);
+  o.jpos = put(buf, h.join_pos);
+  o.K = put(buf, h.K);
+  o.n = put(buf, h.n);
+  o.thr1 = put(buf, h.thr1);
+  o.thr2 = put(buf, h.thr2);
+  o.LF = put(buf, h.LF);
+  o.D = put(buf, h.D);
+  o.H = put(buf, h.H32);
+  o.A = put(buf, h.A32);
+  o.B = put(buf, h.B32);
+  o.err = put(buf, std::vector<int>(4, 0));
+  o.ctr = put(buf, std::vector<uint32_t>(kCounterRing, 0));
+  o.stats = put(buf, std::vector<unsigned long long>(16, 0));
+  buf.resize((buf.size() + 255) & ~(size_t)255);
   j->arena_bytes = buf.size();
   j->device = dev;
-  unsigned char *b = (unsigned char *)j->arena;
   DevJob &d = j->dev;
+  d = DevJob{};
   d.N = (uint32_t)h.N;
   d.N1 = (uint32_t)h.l1.ids.size();
   d.N2 = (uint32_t)h.l2.ids.size();
@@ -154,31 +169,46 @@ int upload_job(dto_job *j) {
   d.dup = h.dup ? 1u : 0u;
   d.kle = h.kle ? 1u : 0u;
   d.margin = h.margin;
-  d.bin1 = (const uint16_t *)(b + o_bin1);
-  d.bin2 = (const uint16_t *)(b + o_bin2);
-  d.join = (const int32_t *)(b + o_join);
-  d.join_off = (const uint32_t *)(b + o_joff);
-  d.join_pos = (const uint32_t *)(b + o_jpos);
-  d.K = (const uint32_t *)(b + o_K);
-  d.n = (const uint32_t *)(b + o_n);
-  d.thr1 = (const uint32_t *)(b + o_t1);
-  d.thr2 = (const uint32_t *)(b + o_t2);
-  d.LF = (const double *)(b + o_LF);
-  d.D = (const double *)(b + o_D);
-  d.H32 = (const float *)(b + o_H);
-  d.A32 = (const float *)(b + o_A);
-  d.B32 = (const float *)(b + o_B);
-  j->d_err = (int *)(b + o_err);
-  j->d_ctr = (uint32_t *)(b + o_ctr);
-  d.stats = (unsigned long long *)(b + o_stats);
-  /* LDS: keep the screen table resident when it fits */
-  const size_t lim = pr.lds_per_block;
-  LdsLayout with_h = make_layout(h, kNT / 64, true, lim), without_h = make_layout(h, kNT / 64, false, lim);
-  if (without_h.total > lim)
-    return fail(DTO_ERR_TOO_LARGE, "job needs " + std::to_string(without_h.total) + " B of LDS per workgroup, device has " +
-                                       std::to_string(lim));
-  j->lo_plain = without_h;
-  j->lo_screen = with_h.total <= lim ? with_h : without_h;
+  return DTO_OK;
+}
+
+void bind_job(dto_job *j, unsigned char *b) {
+  const ArenaOffsets &o = j->off;
+  DevJob &d = j->dev;
+  d.bin1 = (const uint16_t *)(b + o.bin1);
+  d.bin2 = (const uint16_t *)(b + o.bin2);
+  d.join = (const int32_t *)(b + o.join);
+  d.join_off = (const uint32_t *)(b + o.joff);
+  d.join_pos = (const uint32_t *)(b + o.jpos);
+  d.K = (const uint32_t *)(b + o.K);
+  d.n = (const uint32_t *)(b + o.n);
+  d.thr1 = (const uint32_t *)(b + o.thr1);
+  d.thr2 = (const uint32_t *)(b + o.thr2);
+  d.LF = (const double *)(b + o.LF);
+  d.D = (const double *)(b + o.D);
+  d.H32 = (const float *)(b + o.H);
+  d.A32 = (const float *)(b + o.A);
+  d.B32 = (const float *)(b + o.B);
+  j->d_err = (int *)(b + o.err);
+  j->d_ctr = (uint32_t *)(b + o.ctr);
+  d.stats = (unsigned long long *)(b + o.stats);
+  j->resident = true;
+}
+
+int upload_job(dto_job *j) {
+  int rc = prepare_job(j);
+  if (rc) return rc;
+  DevProps pr;
+  if ((rc = device_props(j->device, &pr))) return rc;
+  if (j->arena) {
+    (void)hipFree(j->arena);
+    j->arena = nullptr;
+  }
+  j->resident = false;
+  HIP_TRY(hipMalloc(&j->arena, j->image.size()));
+  HIP_TRY(hipMemcpy(j->arena, j->image.data(), j->image.size(), hipMemcpyHostToDevice));
+  bind_job(j, (unsigned char *)j->arena);
+  std::vector<unsigned char>().swap(j->image); /* the device copy is the job's now */
   /* global pair staging for layouts whose pairs do not fit in LDS */
   if (j->ws) {
     (void)hipFree(j->ws);
@@ -186,14 +216,45 @@ int upload_job(dto_job *j) {
   }
   if (!j->lo_plain.pairs_in_lds || !j->lo_screen.pairs_in_lds) {
     j->ws_slots = (uint32_t)pr.cus * 4u;
-    HIP_TRY(hipMalloc(&j->ws, (size_t)j->ws_slots * std::max<uint32_t>(d.N2, 1) * 4));
+    HIP_TRY(hipMalloc(&j->ws, (size_t)j->ws_slots * std::max<uint32_t>(j->dev.N2, 1) * sizeof(uint16_t)));
+    if (!j->ws_ev) HIP_TRY(hipEventCreateWithFlags(&j->ws_ev, hipEventDisableTiming));
   }
   return DTO_OK;
+}
+
+int ensure_resident(dto_job *j) {
+  if (j->resident) return DTO_OK;
+  return upload_job(j);
 }
 
 /* ------------------------------------------------------------------ */
 /* launches                                                              */
 /* ------------------------------------------------------------------ */
+/* Launches that stage their pairs in the job's global buffer (lists too long
+ * for the LDS) run one after another, whatever their streams: each waits for
+ * the previous one's completion event.  Inside a stream capture the wait is
+ * left out (a capture cannot wait on work recorded outside it), so captured
+ * launches of one such job must not overlap. */
+struct StagingOrder {
+  dto_job *j;
+  hipStream_t st;
+  bool on = false;
+  StagingOrder(const dto_job *job, const LdsLayout &lo, hipStream_t s) : j(const_cast<dto_job *>(job)), st(s) {
+    if (lo.pairs_in_lds || !j->ws_ev) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) (void)hipGetLastError();
+    if (cs != hipStreamCaptureStatusNone) return;
+    j->ws_mu.lock();
+    on = true;
+    if (j->ws_used) (void)hipStreamWaitEvent(st, j->ws_ev, 0);
+  }
+  ~StagingOrder() {
+    if (!on) return;
+    if (hipEventRecord(j->ws_ev, st) == hipSuccess) j->ws_used = true;
+    j->ws_mu.unlock();
+  }
+};
+
 template <bool E, bool HL, bool PL, bool PA, bool RC>
 static hipError_t launch_opt(const dto_job *j, const LdsLayout &lo, uint32_t grid, hipStream_t st,
                              const uint32_t *i1, const uint32_t *i2, uint64_t n_perm, uint64_t base, uint64_t seed,
@@ -202,6 +263,7 @@ static hipError_t launch_opt(const dto_job *j, const LdsLayout &lo, uint32_t gri
   (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lo.total);
   /* a slot of the job's counter ring for this launch, zeroed on its stream */
   uint32_t *ctr = j->d_ctr + (const_cast<dto_job *>(j)->ctr_next.fetch_add(1u) % kCounterRing);
+  StagingOrder order(j, lo, st);
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, ctr);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kNT), lo.total, st, j->dev, lo, i1, i2, n_perm, base, seed, job_id,
                      permuted, out, j->d_err, j->ws, ctr);
@@ -302,6 +364,33 @@ struct DevBuf {
     if (p) (void)hipFree(p);
   }
 };
+
+/* The debug grid of one permutation on a stream: overlap counts into kcnt
+ * (T1 * T2 words), then every cell's exact record into out; an out-of-range
+ * permutation index sets *err. */
+hipError_t dto::launch_grid(dto_job *job, const uint32_t *d1, const uint32_t *d2, uint32_t *kcnt, dto_record *out,
+                              int *err, hipStream_t st) {
+  const size_t cells = (size_t)job->dev.T1 * job->dev.T2;
+  const LdsLayout &lo = job->lo_plain;
+  if (lo.pairs_in_lds) {
+    (void)hipFuncSetAttribute((const void *)k_grid_counts<kNT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lo.total);
+    hipLaunchKernelGGL((k_grid_counts<kNT, true>), dim3(1), dim3(kNT), lo.total, st, job->dev, lo, d1, d2, kcnt, err,
+                       job->ws);
+  } else {
+    (void)hipFuncSetAttribute((const void *)k_grid_counts<kNT, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lo.total);
+    StagingOrder order(job, lo, st);
+    hipLaunchKernelGGL((k_grid_counts<kNT, false>), dim3(1), dim3(kNT), lo.total, st, job->dev, lo, d1, d2, kcnt,
+                       err, job->ws);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int32_t permuted = (d1 || d2) ? 1 : 0;
+  const uint32_t g = (uint32_t)std::min<size_t>((cells + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_grid_pvalues, dim3(g), dim3(256), 0, st, job->dev, (const uint32_t *)kcnt, permuted, out);
+  return hipGetLastError();
+}
 
 /* ------------------------------------------------------------------ */
 /* C ABI                                                                  */
@@ -443,6 +532,7 @@ int dto_internal_shuffle_device(uint64_t seed, uint32_t job, uint32_t perm, uint
  * the reverse sweep's first-term test */
 int dto_internal_stats(dto_job *job, uint64_t *out, int reset) {
   if (!job || !out) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  if (!DTO_STATS) return fail(DTO_ERR_INVALID_ARGUMENT, "the library was built without DTO_STATS (make STATS=1)");
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(out, job->dev.stats, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost));
   if (reset) HIP_TRY(hipMemset(job->dev.stats, 0, 16 * sizeof(uint64_t)));
@@ -462,8 +552,8 @@ int dto_job_create(const char *const *ids1, const uint32_t *ranks1, uint64_t n1,
   if (!out) return fail(DTO_ERR_INVALID_ARGUMENT, "null output");
   *out = nullptr;
   dto_job *j = new dto_job();
-  int rc = build_host_job(ids1, ranks1, n1, ids2, ranks2, n2, bg, nbg, flags, j->h);
-  if (!rc) rc = upload_job(j);
+  int rc = build_host_job(ids1, ranks1, n1, ids2, ranks2, n2, bg, nbg, flags & ~DTO_DEFER_UPLOAD, j->h);
+  if (!rc) rc = (flags & DTO_DEFER_UPLOAD) ? prepare_job(j) : upload_job(j);
   if (rc) {
     dto_job_destroy(j);
     return rc;
@@ -507,15 +597,17 @@ int dto_job_create_from_files(const char *path1, const char *path2, const char *
 
 int dto_job_set_thresholds(dto_job *job, const uint32_t *thr1, uint32_t t1, const uint32_t *thr2, uint32_t t2) {
   if (!job || (t1 && !thr1) || (t2 && !thr2)) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  if (job->resident && !job->arena) return fail(DTO_ERR_INVALID_ARGUMENT, "the job belongs to a server");
+  const bool own = job->resident;
   HostJob saved = job->h;
   job->h.thr1.assign(thr1, thr1 + t1);
   job->h.thr2.assign(thr2, thr2 + t2);
   int rc = job->h.derive();
-  if (!rc) rc = upload_job(job);
+  if (!rc) rc = own ? upload_job(job) : prepare_job(job);
   if (rc) {
     std::string msg = dto_last_error();
     job->h = saved;
-    (void)upload_job(job);
+    (void)(own ? upload_job(job) : prepare_job(job));
     return fail(rc, msg);
   }
   return DTO_OK;
@@ -546,12 +638,15 @@ void dto_job_destroy(dto_job *job) {
   if (!job) return;
   if (job->arena) (void)hipFree(job->arena);
   if (job->ws) (void)hipFree(job->ws);
+  if (job->ws_ev) (void)hipEventDestroy(job->ws_ev);
   delete job;
 }
 
 int dto_run_device(dto_job *job, uint64_t n_perm, uint64_t perm_base, uint64_t seed, const uint32_t *d_idx1,
                    const uint32_t *d_idx2, uint32_t mode, dto_record *d_out, void *stream) {
   if (!job || (!d_out && n_perm)) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  int rc;
+  if ((rc = ensure_resident(job))) return rc;
   bool expl = d_idx1 || d_idx2;
   return launch_perms(job, mode, expl, d_idx1, d_idx2, n_perm, perm_base, seed, 0, 1, d_out, (hipStream_t)stream);
 }
@@ -559,6 +654,7 @@ int dto_run_device(dto_job *job, uint64_t n_perm, uint64_t perm_base, uint64_t s
 int dto_optimize(dto_job *job, const uint32_t *idx1, const uint32_t *idx2, uint32_t mode, dto_record *out) {
   if (!job || !out) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
   int rc;
+  if ((rc = ensure_resident(job))) return rc;
   if ((rc = validate_perms(idx1, 1, job->dev.N1)) || (rc = validate_perms(idx2, 1, job->dev.N2))) return rc;
   DevBuf di, dr;
   const size_t n1 = job->dev.N1, n2 = job->dev.N2;
@@ -582,6 +678,7 @@ int dto_optimize(dto_job *job, const uint32_t *idx1, const uint32_t *idx2, uint3
 int dto_process_threshold_pairs(dto_job *job, const uint32_t *idx1, const uint32_t *idx2, dto_record *out) {
   if (!job || !out) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
   int rc;
+  if ((rc = ensure_resident(job))) return rc;
   if ((rc = validate_perms(idx1, 1, job->dev.N1)) || (rc = validate_perms(idx2, 1, job->dev.N2))) return rc;
   const size_t n1 = job->dev.N1, n2 = job->dev.N2, cells = (size_t)job->dev.T1 * job->dev.T2;
   DevBuf di, dk, dr;
@@ -597,27 +694,11 @@ int dto_process_threshold_pairs(dto_job *job, const uint32_t *idx1, const uint32
     d2 = (uint32_t *)di.p + n1 + 1;
     HIP_TRY(hipMemcpy(d2, idx2, n2 * 4, hipMemcpyHostToDevice));
   }
-  const LdsLayout &lo = job->lo_plain;
-  if (lo.pairs_in_lds) {
-    (void)hipFuncSetAttribute((const void *)k_grid_counts<kNT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lo.total);
-    hipLaunchKernelGGL((k_grid_counts<kNT, true>), dim3(1), dim3(kNT), lo.total, 0, job->dev, lo, d1, d2,
-                       (uint32_t *)dk.p, job->d_err, job->ws);
-  } else {
-    (void)hipFuncSetAttribute((const void *)k_grid_counts<kNT, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lo.total);
-    hipLaunchKernelGGL((k_grid_counts<kNT, false>), dim3(1), dim3(kNT), lo.total, 0, job->dev, lo, d1, d2,
-                       (uint32_t *)dk.p, job->d_err, job->ws);
-  }
-  HIP_TRY(hipGetLastError());
-  int32_t permuted = (idx1 || idx2) ? 1 : 0;
-  uint32_t g = (uint32_t)std::min<size_t>((cells + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_grid_pvalues, dim3(g), dim3(256), 0, 0, job->dev, (const uint32_t *)dk.p, permuted,
-                     (dto_record *)dr.p);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(launch_grid(job, d1, d2, (uint32_t *)dk.p, (dto_record *)dr.p, job->d_err, 0));
   HIP_TRY(hipMemcpy(out, dr.p, cells * sizeof(dto_record), hipMemcpyDeviceToHost));
   return check_err_flag(job);
 }
+
 
 int dto_empirical_pvalue(const dto_record *unpermuted, const dto_record *perm, uint64_t n_perm, dto_result *result) {
   if (!unpermuted || !result || (n_perm && !perm)) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
@@ -639,6 +720,7 @@ int dto_run(dto_job *job, uint64_t n_perm, uint64_t seed, const uint32_t *idx1, 
   if ((idx1 == nullptr) != (idx2 == nullptr))
     return fail(DTO_ERR_INVALID_ARGUMENT, "give explicit permutations for both lists or neither");
   int rc;
+  if ((rc = ensure_resident(job))) return rc;
   const uint64_t n1 = job->dev.N1, n2 = job->dev.N2;
   if ((rc = validate_perms(idx1, n_perm, n1)) || (rc = validate_perms(idx2, n_perm, n2))) return rc;
   DevBuf dr;

@@ -8,11 +8,14 @@
  * from a device queue until it is empty.  A batch's launch sequence (queue
  * reset + persistent kernel) is captured once in a hipGraph and replayed.
  *
- * The server does continuous batching: requests are admitted while their HBM
- * footprint fits the budget, every scheduling slice takes up to
- * perms_per_slice tasks from every active request, and two slices are kept in
- * flight on two streams so requests that arrive while others run join the
- * next slice and overlap with them on the device.
+ * The server does continuous batching: requests are admitted, oldest first,
+ * while their tables and records fit a device pool of hbm_budget_bytes (one
+ * allocation at creation; admission and completion only move regions of it,
+ * so the poll loop never calls hipMalloc or hipFree).  Requests are grouped
+ * by the launch shape their job needs alone; a slice takes up to
+ * perms_per_slice tasks from every active request of one shape class, and up
+ * to four slices are in flight on four streams, so requests that arrive while
+ * others run join the next slice and overlap with them on the device.
  */
 #include <hip/hip_runtime.h>
 #include <string.h>
@@ -20,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <map>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -37,7 +41,7 @@ namespace {
  * screen table in LDS or not, the pair staging in LDS or global memory. */
 template <int NT, bool HL, bool PL, bool RC>
 hipError_t launch_batch_t(uint32_t grid, uint32_t lds, hipStream_t st, const BatchJob *jobs, const WorkItem *items,
-                          uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, uint32_t *ws,
+                          uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, uint16_t *ws,
                           uint32_t ws_stride) {
   auto kern = k_batch<NT, HL, PL, RC>;
   (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -50,7 +54,7 @@ hipError_t launch_batch_t(uint32_t grid, uint32_t lds, hipStream_t st, const Bat
 template <int NT>
 hipError_t launch_batch_n(bool hl, bool pl, bool rc, uint32_t grid, uint32_t lds, hipStream_t st,
                           const BatchJob *jobs, const WorkItem *items, uint32_t n_items, uint64_t seed,
-                          uint32_t *counter, int *err, uint32_t *ws, uint32_t ws_stride) {
+                          uint32_t *counter, int *err, uint16_t *ws, uint32_t ws_stride) {
   if (hl)
     return rc ? launch_batch_t<NT, true, true, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride)
               : launch_batch_t<NT, true, true, false>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride);
@@ -73,7 +77,7 @@ struct Shape {
 };
 
 hipError_t launch_batch(const Shape &sh, uint32_t grid, hipStream_t st, const BatchJob *jobs, const WorkItem *items,
-                        uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, uint32_t *ws) {
+                        uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, uint16_t *ws) {
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, counter, err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -143,7 +147,7 @@ struct dto_batch {
   uint32_t *d_counter = nullptr;
   int *d_err = nullptr;
   dto_record *d_out = nullptr;
-  uint32_t *d_ws = nullptr; /* global pair staging when the LDS cannot hold the pairs */
+  uint16_t *d_ws = nullptr; /* global pair staging when the LDS cannot hold the pairs */
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t own_stream = nullptr;
@@ -200,7 +204,7 @@ static int batch_plan(dto_batch *b) {
   HIP_TRY(hipMemcpy(b->d_jobs, hj.data(), hj.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
   if (!items.empty()) HIP_TRY(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
   b->grid = grid_for(b->cus, b->shape.lds, b->shape.nt, b->n_items);
-  if (!b->shape.pl) HIP_TRY(hipMalloc(&b->d_ws, (size_t)b->grid * b->shape.max_n2 * 4));
+  if (!b->shape.pl) HIP_TRY(hipMalloc(&b->d_ws, (size_t)b->grid * b->shape.max_n2 * sizeof(uint16_t)));
   return DTO_OK;
 }
 
@@ -224,6 +228,10 @@ int dto_batch_create(dto_job *const *jobs, const uint64_t *n_perm, uint64_t n_jo
     if (!jobs[i] || n_perm[i] >= 0xFFFFFFFFull) {
       delete b;
       return fail(DTO_ERR_INVALID_ARGUMENT, "bad job or permutation count");
+    }
+    if ((rc = ensure_resident(jobs[i]))) {
+      delete b;
+      return rc;
     }
     b->jobs.push_back(jobs[i]);
     b->n_perm.push_back(n_perm[i]);
@@ -348,15 +356,93 @@ void dto_batch_destroy(dto_batch *b) {
 /* ------------------------------------------------------------------ */
 namespace {
 
+constexpr int kSlots = 4;         /* slices in flight, one stream each */
+constexpr uint32_t kSliceChunk = 4; /* tasks per work item in a slice */
+constexpr size_t kPoolAlign = 256;
+
+/* First-fit allocator over the server's one device block: admission,
+ * completion and slice staging take and return regions without any device
+ * allocation (hipMalloc / hipFree would synchronise the device). */
+struct Pool {
+  unsigned char *base = nullptr;
+  size_t size = 0, used = 0;
+  std::map<size_t, size_t> free_; /* offset -> bytes, coalesced */
+
+  void init(void *b, size_t n) {
+    base = (unsigned char *)b;
+    size = n;
+    used = 0;
+    free_.clear();
+    if (n) free_[0] = n;
+  }
+  bool alloc(size_t bytes, size_t *off) {
+    bytes = (std::max<size_t>(bytes, 1) + kPoolAlign - 1) & ~(kPoolAlign - 1);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      *off = it->first;
+      const size_t rest = it->second - bytes, at = it->first + bytes;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      used += bytes;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 1) + kPoolAlign - 1) & ~(kPoolAlign - 1);
+    used -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+/* Launch class of a job on its own: screen table in LDS, pairs in LDS,
+ * column constants in registers, and workgroups per CU its LDS allows.
+ * Requests of one class share slices; a large request never moves small
+ * ones onto its slower shape or its lower occupancy. */
+struct ShapeClass {
+  bool hl, pl, rc;
+  uint32_t per_cu;
+  bool operator==(const ShapeClass &o) const { return hl == o.hl && pl == o.pl && rc == o.rc && per_cu == o.per_cu; }
+};
+
+ShapeClass class_of(const dto_job *j) {
+  const LdsLayout &L = j->lo_screen;
+  ShapeClass c;
+  c.hl = L.h_in_lds != 0;
+  c.pl = L.pairs_in_lds != 0;
+  c.rc = (j->dev.T2w + 63) / 64 <= kSegWords;
+  c.per_cu = std::min<uint32_t>(4, std::max<uint32_t>(1, (160u * 1024u) / std::max<uint32_t>(L.total, 1)));
+  return c;
+}
+
+constexpr size_t kNoRegion = ~(size_t)0;
+
 struct Request {
   uint64_t id = 0;
   dto_job *job = nullptr;
   uint64_t n_perm = 0;
   uint64_t next_task = 0; /* next task to schedule (0 = unpermuted) */
   uint64_t done_tasks = 0;
+  uint32_t inflight = 0; /* slices in flight that hold some of its tasks */
+  bool failed = false;
+  bool admitted = false;
   double t_submit = 0.0;
+  ShapeClass cls{};
   dto_record *d_out = nullptr;
-  size_t bytes = 0;
+  size_t out_off = kNoRegion, out_bytes = 0;
+  size_t job_off = kNoRegion, job_bytes = 0; /* the job's tables in the pool (deferred jobs) */
 };
 
 struct Part {
@@ -364,20 +450,19 @@ struct Part {
   uint32_t t0, count;
 };
 
-struct Slice {
+struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t ev = nullptr;
   BatchJob *d_jobs = nullptr;
   WorkItem *d_items = nullptr;
   BatchJob *h_jobs = nullptr; /* pinned staging */
   WorkItem *h_items = nullptr;
+  int *h_err = nullptr; /* pinned: the launch's error word, copied back on the stream */
   uint32_t *d_counter = nullptr;
   int *d_err = nullptr;
-  uint32_t *d_ws = nullptr;
-  size_t ws_words = 0;
+  size_t ws_off = kNoRegion, ws_bytes = 0;
   std::vector<Part> parts;
   bool inflight = false;
-  uint32_t cap_jobs = 0, cap_items = 0;
 };
 
 }  // namespace
@@ -390,77 +475,144 @@ struct dto_server {
   uint64_t next_id = 1;
   std::deque<Request *> queue;
   std::vector<Request *> active;
-  size_t active_bytes = 0;
-  Slice slices[2];
+  Pool pool;
+  void *pool_mem = nullptr;
+  hipStream_t up = nullptr; /* uploads of admitted jobs */
+  hipEvent_t up_ev = nullptr;
+  bool up_any = false;
+  Slot slots[kSlots];
+  uint32_t cap_items = 0;
+  std::vector<dto_job *> graveyard; /* own-arena jobs, freed when no slice is in flight */
   uint32_t max_queue = 1u << 20;
 };
 
-static int slice_reserve(Slice &s, uint32_t jobs, uint32_t items) {
-  if (jobs > s.cap_jobs) {
-    if (s.d_jobs) (void)hipFree(s.d_jobs);
-    if (s.h_jobs) (void)hipHostFree(s.h_jobs);
-    HIP_TRY(hipMalloc(&s.d_jobs, jobs * sizeof(BatchJob)));
-    HIP_TRY(hipHostMalloc(&s.h_jobs, jobs * sizeof(BatchJob), hipHostMallocDefault));
-    s.cap_jobs = jobs;
+static void server_free_request(dto_server *sv, Request *r) {
+  if (r->out_off != kNoRegion) sv->pool.release(r->out_off, r->out_bytes);
+  if (r->job_off != kNoRegion) sv->pool.release(r->job_off, r->job_bytes);
+  if (r->job) {
+    if (r->job->arena || r->job->ws)
+      sv->graveyard.push_back(r->job); /* its hipFree would wait for the device */
+    else
+      dto_job_destroy(r->job);
   }
-  if (items > s.cap_items) {
-    if (s.d_items) (void)hipFree(s.d_items);
-    if (s.h_items) (void)hipHostFree(s.h_items);
-    HIP_TRY(hipMalloc(&s.d_items, items * sizeof(WorkItem)));
-    HIP_TRY(hipHostMalloc(&s.h_items, items * sizeof(WorkItem), hipHostMallocDefault));
-    s.cap_items = items;
+  delete r;
+}
+
+static bool server_busy(const dto_server *sv) {
+  for (const Slot &s : sv->slots)
+    if (s.inflight) return true;
+  return false;
+}
+
+/* Admits queued requests, oldest first, while the pool holds their tables
+ * and records; copies a deferred job's tables into its region. */
+static int server_admit(dto_server *sv, std::string *dropped) {
+  bool any = false;
+  while (!sv->queue.empty() && sv->active.size() < sv->cfg.max_active_jobs) {
+    Request *r = sv->queue.front();
+    dto_job *j = r->job;
+    const size_t out_b = (r->n_perm + 1) * sizeof(dto_record);
+    const size_t job_b = j->resident ? 0 : j->image.size();
+    const size_t need = ((out_b + kPoolAlign - 1) & ~(kPoolAlign - 1)) + ((job_b + kPoolAlign - 1) & ~(kPoolAlign - 1));
+    if (need > sv->pool.size) {
+      /* can never fit: drop it rather than block the queue */
+      sv->queue.pop_front();
+      *dropped += (dropped->empty() ? "" : ", ") + std::to_string(r->id);
+      server_free_request(sv, r);
+      continue;
+    }
+    size_t oo, jo = kNoRegion;
+    if (!sv->pool.alloc(out_b, &oo)) break;
+    if (job_b && !sv->pool.alloc(job_b, &jo)) {
+      sv->pool.release(oo, out_b);
+      break;
+    }
+    r->out_off = oo;
+    r->out_bytes = out_b;
+    r->d_out = (dto_record *)(sv->pool.base + oo);
+    if (job_b) {
+      r->job_off = jo;
+      r->job_bytes = job_b;
+      HIP_TRY(hipMemcpyAsync(sv->pool.base + jo, j->image.data(), job_b, hipMemcpyHostToDevice, sv->up));
+      bind_job(j, sv->pool.base + jo);
+      any = true;
+    }
+    r->admitted = true;
+    sv->active.push_back(r);
+    sv->queue.pop_front();
+  }
+  if (any) {
+    HIP_TRY(hipEventRecord(sv->up_ev, sv->up));
+    sv->up_any = true;
   }
   return DTO_OK;
 }
 
-/* Builds and launches one scheduling slice on slice s. */
-static int server_launch_slice(dto_server *sv, Slice &s) {
+/* Builds and launches one slice on slot s: every active request of the
+ * class of the oldest request with unscheduled tasks gives up to
+ * perms_per_slice of its tasks. */
+static int server_launch_slice(dto_server *sv, Slot &s) {
   s.parts.clear();
+  const Request *oldest = nullptr;
+  for (Request *r : sv->active)
+    if (!r->failed && r->next_task < r->n_perm + 1 && (!oldest || r->t_submit < oldest->t_submit)) oldest = r;
+  if (!oldest) return DTO_OK;
+  const ShapeClass cls = oldest->cls;
+  uint32_t n_items = 0, lds = 0, max_n2 = 1;
   for (Request *r : sv->active) {
-    uint64_t tasks = r->n_perm + 1;
-    if (r->next_task >= tasks) continue;
-    uint32_t c = (uint32_t)std::min<uint64_t>(sv->cfg.perms_per_slice, tasks - r->next_task);
+    if (r->failed || !(r->cls == cls) || r->next_task >= r->n_perm + 1) continue;
+    const uint32_t c = (uint32_t)std::min<uint64_t>(sv->cfg.perms_per_slice, r->n_perm + 1 - r->next_task);
+    const uint32_t its = (c + kSliceChunk - 1) / kSliceChunk;
+    if (n_items + its > sv->cap_items || s.parts.size() >= sv->cfg.max_active_jobs) break;
     s.parts.push_back(Part{r, (uint32_t)r->next_task, c});
-    r->next_task += c;
+    n_items += its;
+    lds = std::max(lds, r->job->lo_screen.total);
+    max_n2 = std::max(max_n2, r->job->dev.N2);
   }
   if (s.parts.empty()) return DTO_OK;
-  std::vector<dto_job *> jobs;
-  for (Part &p : s.parts) jobs.push_back(p.req->job);
-  Shape shape;
-  int rc = plan_shape(jobs, 512, sv->lds_limit, shape);
-  if (rc) return rc;
-  const uint32_t chunk = 4;
-  uint32_t n_items = 0;
-  for (Part &p : s.parts) n_items += (p.count + chunk - 1) / chunk;
-  if ((rc = slice_reserve(s, (uint32_t)s.parts.size(), n_items))) return rc;
+  const uint32_t grid = grid_for(sv->cus, lds, kNT, n_items);
+  if (!cls.pl) {
+    s.ws_bytes = (size_t)grid * max_n2 * sizeof(uint16_t);
+    if (!sv->pool.alloc(s.ws_bytes, &s.ws_off)) {
+      s.parts.clear(); /* no room for the staging now: try again later */
+      s.ws_off = kNoRegion;
+      return DTO_OK;
+    }
+  }
   uint32_t it = 0;
   for (size_t i = 0; i < s.parts.size(); ++i) {
     Part &p = s.parts[i];
     BatchJob &bj = s.h_jobs[i];
     bj.J = p.req->job->dev;
-    bj.lo = shape.lo[i];
+    bj.lo = p.req->job->lo_screen;
     bj.out = p.req->d_out;
     bj.n_perm = (uint32_t)p.req->n_perm;
     bj.job_id = (uint32_t)(p.req->id & 0xFFFFFFFFu);
-    for (uint32_t t0 = p.t0; t0 < p.t0 + p.count; t0 += chunk) {
+    for (uint32_t t0 = p.t0; t0 < p.t0 + p.count; t0 += kSliceChunk) {
       WorkItem &w = s.h_items[it++];
       w.job = (uint32_t)i;
       w.t0 = t0;
-      w.count = std::min(chunk, p.t0 + p.count - t0);
+      w.count = std::min(kSliceChunk, p.t0 + p.count - t0);
       w.pad = 0;
     }
+    p.req->next_task += p.count;
+    p.req->inflight++;
   }
+  /* jobs admitted since the last slice: their tables must have arrived */
+  if (sv->up_any) HIP_TRY(hipStreamWaitEvent(s.st, sv->up_ev, 0));
   HIP_TRY(hipMemcpyAsync(s.d_jobs, s.h_jobs, s.parts.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s.st));
   HIP_TRY(hipMemcpyAsync(s.d_items, s.h_items, n_items * sizeof(WorkItem), hipMemcpyHostToDevice, s.st));
-  uint32_t grid = grid_for(sv->cus, shape.lds, 512, n_items);
-  if (!shape.pl && (size_t)grid * shape.max_n2 > s.ws_words) {
-    if (s.d_ws) (void)hipFree(s.d_ws);
-    s.ws_words = (size_t)grid * shape.max_n2;
-    HIP_TRY(hipMalloc(&s.d_ws, s.ws_words * 4));
-  }
-  hipError_t e = launch_batch(shape, grid, s.st, s.d_jobs, s.d_items, n_items, sv->cfg.seed, s.d_counter, s.d_err,
-                              s.d_ws);
+  Shape shape;
+  shape.nt = kNT;
+  shape.hl = cls.hl;
+  shape.pl = cls.pl;
+  shape.rc = cls.rc;
+  shape.lds = lds;
+  shape.max_n2 = max_n2;
+  uint16_t *ws = cls.pl ? nullptr : (uint16_t *)(sv->pool.base + s.ws_off);
+  hipError_t e = launch_batch(shape, grid, s.st, s.d_jobs, s.d_items, n_items, sv->cfg.seed, s.d_counter, s.d_err, ws);
   if (e != hipSuccess) return fail(DTO_ERR_DEVICE, std::string("slice launch: ") + hipGetErrorString(e));
+  HIP_TRY(hipMemcpyAsync(s.h_err, s.d_err, sizeof(int), hipMemcpyDeviceToHost, s.st));
   HIP_TRY(hipEventRecord(s.ev, s.st));
   s.inflight = true;
   return DTO_OK;
@@ -485,14 +637,28 @@ int dto_server_create(const dto_server_config *cfg, dto_server **out) {
   }
   sv->cus = pr.cus;
   sv->lds_limit = pr.lds_per_block;
-  for (Slice &s : sv->slices) {
-    if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess || hipMalloc(&s.d_counter, 64) != hipSuccess ||
-        hipMalloc(&s.d_err, 64) != hipSuccess) {
-      dto_server_destroy(sv);
-      return fail(DTO_ERR_DEVICE, "server stream setup failed");
-    }
+  sv->cap_items = sv->cfg.max_active_jobs * ((sv->cfg.perms_per_slice + kSliceChunk - 1) / kSliceChunk);
+  bool ok = hipMalloc(&sv->pool_mem, sv->cfg.hbm_budget_bytes) == hipSuccess &&
+            hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&sv->up_ev, hipEventDisableTiming) == hipSuccess;
+  for (Slot &s : sv->slots) {
+    ok = ok && hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) == hipSuccess &&
+         hipMalloc(&s.d_counter, 64) == hipSuccess && hipMalloc(&s.d_err, 64) == hipSuccess &&
+         hipMalloc(&s.d_jobs, (size_t)sv->cfg.max_active_jobs * sizeof(BatchJob)) == hipSuccess &&
+         hipMalloc(&s.d_items, (size_t)sv->cap_items * sizeof(WorkItem)) == hipSuccess &&
+         hipHostMalloc(&s.h_jobs, (size_t)sv->cfg.max_active_jobs * sizeof(BatchJob), hipHostMallocDefault) ==
+             hipSuccess &&
+         hipHostMalloc(&s.h_items, (size_t)sv->cap_items * sizeof(WorkItem), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&s.h_err, 64, hipHostMallocDefault) == hipSuccess;
   }
+  if (!ok) {
+    (void)hipGetLastError();
+    dto_server_destroy(sv);
+    return fail(DTO_ERR_DEVICE, "server setup failed (device pool of " + std::to_string(cfg->hbm_budget_bytes) +
+                                    " B, streams, slice buffers)");
+  }
+  sv->pool.init(sv->pool_mem, sv->cfg.hbm_budget_bytes);
   *out = sv;
   return DTO_OK;
 }
@@ -500,12 +666,15 @@ int dto_server_create(const dto_server_config *cfg, dto_server **out) {
 int dto_server_submit(dto_server *sv, dto_job *job, uint64_t n_perm, uint64_t *request_id) {
   if (!sv || !job || !request_id) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
   if (sv->queue.size() >= sv->max_queue) return fail(DTO_ERR_BUSY, "request queue full");
+  if (n_perm >= 0xFFFFFFFFull) return fail(DTO_ERR_TOO_LARGE, "too many permutations in one request");
+  if (job->resident && !job->arena) return fail(DTO_ERR_INVALID_ARGUMENT, "the job already belongs to a server");
+  if (job->device != sv->device) return fail(DTO_ERR_INVALID_ARGUMENT, "the job was built for another device");
   Request *r = new Request();
   r->id = sv->next_id++;
   r->job = job;
   r->n_perm = n_perm;
   r->t_submit = now_s();
-  r->bytes = job->arena_bytes + (n_perm + 1) * sizeof(dto_record);
+  r->cls = class_of(job);
   sv->queue.push_back(r);
   *request_id = r->id;
   return DTO_OK;
@@ -520,66 +689,72 @@ int dto_server_pending(const dto_server *sv, uint64_t *queued, uint64_t *active)
 
 int dto_server_poll(dto_server *sv, uint32_t timeout_ms, uint64_t *ids, dto_result *results, double *latency_s,
                     uint32_t cap, uint32_t *n_done) {
-  if (!sv || !n_done) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
+  if (!sv || !n_done || (cap && !results)) return fail(DTO_ERR_INVALID_ARGUMENT, "null pointer");
   *n_done = 0;
   const double t_end = now_s() + timeout_ms * 1e-3;
+  std::string failed, dropped;
+  std::vector<dto_record> rec;
   for (;;) {
-    /* retire finished slices */
-    for (Slice &s : sv->slices) {
+    /* retire finished slices; a slice that reports an error fails only the
+     * requests it carried */
+    for (Slot &s : sv->slots) {
       if (!s.inflight) continue;
       hipError_t q = hipEventQuery(s.ev);
       if (q == hipErrorNotReady) continue;
       if (q != hipSuccess) return fail(DTO_ERR_DEVICE, std::string("slice: ") + hipGetErrorString(q));
-      int herr = 0;
-      HIP_TRY(hipMemcpy(&herr, s.d_err, sizeof(int), hipMemcpyDeviceToHost));
-      if (herr) return fail(DTO_ERR_BAD_PERMUTATION, "permutation index out of range in slice");
-      for (Part &p : s.parts) p.req->done_tasks += p.count;
+      const int herr = *(volatile int *)s.h_err;
+      for (Part &p : s.parts) {
+        p.req->inflight--;
+        if (herr)
+          p.req->failed = true;
+        else
+          p.req->done_tasks += p.count;
+      }
       s.parts.clear();
-      s.inflight = false;
+      if (s.ws_off != kNoRegion) sv->pool.release(s.ws_off, s.ws_bytes);
+      s.ws_off = kNoRegion;
+      s.inflight = false; /* the next launch on this slot clears its error word */
     }
-    /* complete requests */
+    /* complete (or drop failed) requests */
     for (size_t i = 0; i < sv->active.size() && *n_done < cap;) {
       Request *r = sv->active[i];
-      bool busy = false;
-      for (Slice &s : sv->slices)
-        if (s.inflight)
-          for (Part &p : s.parts) busy |= p.req == r;
-      if (r->done_tasks == r->n_perm + 1 && !busy) {
-        std::vector<dto_record> rec(r->n_perm + 1);
+      if (r->inflight) {
+        ++i;
+        continue;
+      }
+      if (r->failed) {
+        failed += (failed.empty() ? "" : ", ") + std::to_string(r->id);
+      } else if (r->done_tasks == r->n_perm + 1) {
+        rec.resize(r->n_perm + 1);
         HIP_TRY(hipMemcpy(rec.data(), r->d_out, rec.size() * sizeof(dto_record), hipMemcpyDeviceToHost));
         int rc = dto_empirical_pvalue(&rec[0], rec.data() + 1, r->n_perm, &results[*n_done]);
         if (rc) return rc;
         if (ids) ids[*n_done] = r->id;
         if (latency_s) latency_s[*n_done] = now_s() - r->t_submit;
         ++*n_done;
-        (void)hipFree(r->d_out);
-        sv->active_bytes -= r->bytes;
-        dto_job_destroy(r->job);
-        delete r;
-        sv->active.erase(sv->active.begin() + (long)i);
       } else {
         ++i;
+        continue;
       }
+      sv->active.erase(sv->active.begin() + (long)i);
+      server_free_request(sv, r);
     }
-    /* admit queued requests while their footprint fits */
-    while (!sv->queue.empty() && sv->active.size() < sv->cfg.max_active_jobs) {
-      Request *r = sv->queue.front();
-      if (!sv->active.empty() && sv->active_bytes + r->bytes > sv->cfg.hbm_budget_bytes) break;
-      HIP_TRY(hipMalloc(&r->d_out, (r->n_perm + 1) * sizeof(dto_record)));
-      sv->active_bytes += r->bytes;
-      sv->active.push_back(r);
-      sv->queue.pop_front();
+    if (!server_busy(sv) && !sv->graveyard.empty()) {
+      for (dto_job *j : sv->graveyard) dto_job_destroy(j);
+      sv->graveyard.clear();
     }
-    /* keep both slices busy */
-    for (Slice &s : sv->slices)
-      if (!s.inflight) {
-        int rc = server_launch_slice(sv, s);
-        if (rc) return rc;
-      }
+    int rc = server_admit(sv, &dropped);
+    if (rc) return rc;
+    for (Slot &s : sv->slots)
+      if (!s.inflight && (rc = server_launch_slice(sv, s))) return rc;
+    if (!failed.empty() || !dropped.empty()) {
+      std::string msg;
+      if (!failed.empty()) msg += "a device error (out-of-range permutation index or no grid cell) failed requests " + failed;
+      if (!dropped.empty()) msg += (msg.empty() ? "" : "; ") + std::string("requests larger than the HBM budget were dropped: ") + dropped;
+      return fail(failed.empty() ? DTO_ERR_TOO_LARGE : DTO_ERR_DEVICE, msg);
+    }
     if (*n_done > 0 || now_s() >= t_end) return DTO_OK;
-    bool any = false;
-    for (Slice &s : sv->slices) any |= s.inflight;
-    if (!any && sv->queue.empty() && sv->active.empty()) return DTO_OK;
+    if (!server_busy(sv) && sv->queue.empty() && sv->active.empty()) return DTO_OK;
     std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
 }
@@ -587,26 +762,23 @@ int dto_server_poll(dto_server *sv, uint32_t timeout_ms, uint64_t *ids, dto_resu
 void dto_server_destroy(dto_server *sv) {
   if (!sv) return;
   (void)hipDeviceSynchronize();
-  for (Slice &s : sv->slices) {
+  for (Slot &s : sv->slots) {
     if (s.st) (void)hipStreamDestroy(s.st);
     if (s.ev) (void)hipEventDestroy(s.ev);
     if (s.d_jobs) (void)hipFree(s.d_jobs);
     if (s.d_items) (void)hipFree(s.d_items);
     if (s.h_jobs) (void)hipHostFree(s.h_jobs);
     if (s.h_items) (void)hipHostFree(s.h_items);
+    if (s.h_err) (void)hipHostFree(s.h_err);
     if (s.d_counter) (void)hipFree(s.d_counter);
     if (s.d_err) (void)hipFree(s.d_err);
-    if (s.d_ws) (void)hipFree(s.d_ws);
   }
-  for (Request *r : sv->active) {
-    if (r->d_out) (void)hipFree(r->d_out);
-    dto_job_destroy(r->job);
-    delete r;
-  }
-  for (Request *r : sv->queue) {
-    dto_job_destroy(r->job);
-    delete r;
-  }
+  for (Request *r : sv->active) server_free_request(sv, r);
+  for (Request *r : sv->queue) server_free_request(sv, r);
+  for (dto_job *j : sv->graveyard) dto_job_destroy(j);
+  if (sv->up) (void)hipStreamDestroy(sv->up);
+  if (sv->up_ev) (void)hipEventDestroy(sv->up_ev);
+  if (sv->pool_mem) (void)hipFree(sv->pool_mem);
   delete sv;
 }
 

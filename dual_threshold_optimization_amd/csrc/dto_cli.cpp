@@ -88,6 +88,22 @@ static uint32_t max_rank(const std::string &path) {
   return mx;
 }
 
+/* optimize's stderr lines for a task whose minimum p is shared
+ * (src/dto/optimize_main.rs:86-108); {:.15} is fixed notation with 15
+ * decimals, as %.15f prints it. */
+static void tie_warnings(const dto_record &r) {
+  if (DTO_TIES_AT_MIN_P(&r) <= 1) return;
+  fprintf(stderr,
+          "Multiple results with the same minimum p-value (%.15f). Choosing the result with the largest intersection "
+          "size.\n",
+          r.pvalue);
+  if (DTO_TIES_AT_MAX_K(&r) > 1)
+    fprintf(stderr,
+            "Multiple results with the same maximum intersection size (%llu). Choosing an arbitrary result based on "
+            "the order of thresholds.\n",
+            (unsigned long long)r.intersection_size);
+}
+
 int main(int argc, char **argv) {
   std::string l1, l2, bg;
   bool have_bg = false, multi = false, drop = false, have_seed = false;
@@ -201,8 +217,13 @@ int main(int argc, char **argv) {
   if (multi) panic("Multi-node mode requires MPI. Rebuild with the 'mpi' feature enabled.");
   if (!have_seed) seed = (uint64_t)time(nullptr) ^ ((uint64_t)clock() << 32);
   dto_result res;
-  rc = dto_run(job, perms, seed, nullptr, nullptr, mode, &res, nullptr);
+  std::vector<dto_record> perm_rec(perms ? perms : 1);
+  rc = dto_run(job, perms, seed, nullptr, nullptr, mode, &res, perm_rec.data());
   if (rc) panic(dto_last_error());
+  /* every task's optimize warns about its own ties (optimize_main.rs:86-108);
+   * the reference's threads print them in completion order, here task order */
+  tie_warnings(res.unpermuted);
+  for (uint64_t p = 0; p < perms; ++p) tie_warnings(perm_rec[p]);
   char buf[4096];
   if (dto_result_json(&res, buf, sizeof buf) < 0) panic("json formatting failed");
   printf("%s\n", buf);

@@ -42,7 +42,7 @@ struct DevJob {
 /* Byte offsets of the dynamic LDS regions of one workgroup. */
 struct LdsLayout {
   uint32_t h;        /* f32[N+1] screen table (only when h_in_lds); always 0 */
-  uint32_t pairs;    /* u32[N2] (a,b) staging: aliases h; always 0 */
+  uint32_t pairs;    /* u16[N2] row of each list2 position's pair (staging): aliases h; always 0 */
   uint32_t pairs_in_lds;
   uint32_t nb;       /* u16[max(N1,N2)]: list1 bins after the shuffle, then row-sorted list2 bins */
   uint32_t rowstart; /* u32[T1+1] */
@@ -66,11 +66,14 @@ constexpr int kCand0Cap = 128;
 constexpr int kCandCap = 64;
 constexpr int kCand3Cap = 32;
 
-/* The best cell found so far: key (p, -k, i, j), lexicographic minimum. */
+/* The best cell found so far: key (p, -k, i, j), lexicographic minimum,
+ * with the number of cells seen at its p (np) and, of those, at its k (npk):
+ * optimize warns when either is above one. */
 struct BestKey {
   double p;
   uint32_t k;
   uint32_t ij; /* i << 16 | j; 0xFFFFFFFF = none */
+  uint32_t np, npk;
 };
 
 }  // namespace dto

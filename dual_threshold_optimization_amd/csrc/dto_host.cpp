@@ -115,7 +115,7 @@ int build_host_job(const char *const *ids1, const uint32_t *ranks1, uint64_t n1,
                    HostJob &J) {
   if ((n1 && (!ids1 || !ranks1)) || (n2 && (!ids2 || !ranks2)) || (nbg && !bg))
     return fail(DTO_ERR_INVALID_ARGUMENT, "null list pointer");
-  if (flags & ~DTO_DROP_OUTSIDE_BACKGROUND) return fail(DTO_ERR_INVALID_ARGUMENT, "unknown flag");
+  if (flags & ~(DTO_DROP_OUTSIDE_BACKGROUND | DTO_DEFER_UPLOAD)) return fail(DTO_ERR_INVALID_ARGUMENT, "unknown flag");
   RankedList a, b;
   for (uint64_t i = 0; i < n1; ++i) {
     if (!ids1[i]) return fail(DTO_ERR_INVALID_ARGUMENT, "null feature id");

@@ -33,6 +33,12 @@
 
 #pragma clang fp contract(off)
 
+/* Work counters and cycle stamps of the screened kernels (profiling builds
+ * only: make STATS=1); off, they compile to nothing. */
+#ifndef DTO_STATS
+#define DTO_STATS 0
+#endif
+
 namespace dto {
 
 static __constant__ uint64_t kExpTabC[256] = DTO_EXP_TAB_INIT;
@@ -64,20 +70,42 @@ __device__ __forceinline__ bool better(double p1, uint32_t k1, uint32_t ij1, dou
   if (k1 != k2) return k1 > k2;
   return ij1 < ij2;
 }
-__device__ __forceinline__ void take_better(BestKey &b, double p, uint32_t k, uint32_t ij) {
-  if (better(p, k, ij, b.p, b.k, b.ij)) {
-    b.p = p;
-    b.k = k;
-    b.ij = ij;
+/* a <- the better of a and b, with the cells of both counted: cells at the
+ * winning p add up, and so do, of those, the cells at the winning k.  Every
+ * cell enters exactly one key once, so the counts are exact. */
+__device__ __forceinline__ void merge_best(BestKey &a, const BestKey &b) {
+  if (b.ij == kNone) return;
+  if (a.ij == kNone) {
+    a = b;
+    return;
   }
+  if (b.p != a.p) {
+    if (b.p < a.p) a = b;
+    return;
+  }
+  const uint32_t np = a.np + b.np;
+  if (b.k > a.k) {
+    a = b;
+  } else if (b.k == a.k) {
+    a.npk += b.npk;
+    a.ij = b.ij < a.ij ? b.ij : a.ij;
+  }
+  a.np = np;
 }
+__device__ __forceinline__ void take_better(BestKey &b, double p, uint32_t k, uint32_t ij) {
+  merge_best(b, BestKey{p, k, ij, 1u, 1u});
+}
+__device__ __forceinline__ BestKey no_best() { return BestKey{__builtin_inf(), 0u, kNone, 0u, 0u}; }
 
 __device__ __forceinline__ BestKey wave_best(BestKey b) {
   for (int off = 32; off >= 1; off >>= 1) {
-    double p = __shfl_xor(b.p, off);
-    uint32_t k = __shfl_xor(b.k, off);
-    uint32_t ij = __shfl_xor(b.ij, off);
-    take_better(b, p, k, ij);
+    BestKey o;
+    o.p = __shfl_xor(b.p, off);
+    o.k = __shfl_xor(b.k, off);
+    o.ij = __shfl_xor(b.ij, off);
+    o.np = __shfl_xor(b.np, off);
+    o.npk = __shfl_xor(b.npk, off);
+    merge_best(b, o);
   }
   return b;
 }
@@ -155,7 +183,7 @@ __device__ __forceinline__ void block_excl_scan(uint32_t *data, uint32_t len, ui
 
 struct Lds {
   float *h;
-  uint32_t *pairs;
+  uint16_t *pairs;
   uint16_t *nb;
   uint32_t *rowstart, *cursor, *cw;
   float *rowmin;
@@ -166,7 +194,7 @@ struct Lds {
   uint4 *cand3;
   uint64_t *tab;
   float *redU;
-  BestKey *redB; /* [3][W]: zero class, one class, evaluated */
+  BestKey *redB; /* [3][W]: p == 0 class, p == 1 class, evaluated cells */
   uint32_t *redS;
   int *ushare; /* the reverse sweep's best upper bound of the minimum log p, as an ordered key */
 };
@@ -176,7 +204,7 @@ __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo
   /* the screen table and the pair staging start the dynamic region (their
    * layout offsets are always 0), so table addresses are the scaled indices */
   s.h = (float *)smem;
-  s.pairs = (uint32_t *)smem;
+  s.pairs = (uint16_t *)smem;
   s.nb = (uint16_t *)(smem + Lo.nb);
   s.rowstart = (uint32_t *)(smem + Lo.rowstart);
   s.cursor = (uint32_t *)(smem + Lo.cursor);
@@ -291,18 +319,20 @@ __device__ __forceinline__ void shuffle_visit(const Shuffle &s, uint32_t *wsum, 
 }
 
 /* Phases A-B for one permutation: leaves the row-sorted list2 bins in S.nb,
- * row offsets in S.rowstart and zeroed per-wave column counts.  The (a, b) pairs are
- * staged in `pairs` (LDS: the screen-table region, reloaded afterwards when
- * HLDS; or a per-workgroup global buffer when the LDS cannot hold them). */
+ * row offsets in S.rowstart and zeroed per-wave column counts.  The row a of
+ * each list2 position's pair is staged as a u16 in `pairs` (LDS: the
+ * screen-table region, reloaded afterwards when HLDS; or a per-workgroup
+ * global buffer when the LDS cannot hold it); the pair's column b is the
+ * position's own bin, read again from bin2 when the pairs are sorted. */
 template <int NT, bool EXPLICIT, bool HLDS, bool PL>
 __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
                                            const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
-                                           uint32_t *gpairs) {
+                                           uint16_t *gpairs) {
   constexpr int W = NT / 64;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   const uint32_t T1 = J.T1, T2 = J.T2, T2wp = Lo.T2w_pad;
-  uint32_t *lpairs = S.pairs;
-  volatile uint32_t *vpairs = gpairs; /* global staging: bypass the per-CU cache on the way back */
+  uint16_t *lpairs = S.pairs;
+  volatile uint16_t *vpairs = gpairs; /* global staging: bypass the per-CU cache on the way back */
   for (uint32_t t = tid; t <= T1; t += NT) S.rowstart[t] = 0;
   for (uint32_t t = tid; t < W * T2wp; t += NT) S.cw[t] = 0;
   if (tid == 0) *S.ushare = fkey(__builtin_inff()); /* published after the build's barriers */
@@ -349,9 +379,9 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
         a = v < a ? v : a;
       }
     }
-    uint32_t pr = kNone;
+    uint16_t pr = (uint16_t)kBinNone;
     if (a < T1 && b < T2) {
-      pr = (a << 16) | b;
+      pr = (uint16_t)a;
       atomicAdd(&S.rowstart[a], 1u);
     }
     if (PL)
@@ -395,12 +425,28 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
   block_excl_scan<NT>(S.rowstart, T1, S.redS);
   for (uint32_t t = tid; t < T1; t += NT) S.cursor[t] = S.rowstart[t];
   __syncthreads();
-  /* B2: scatter b into row order (overwrites the list1 bins, all read above) */
-  for (uint32_t e = tid; e < J.N2; e += NT) {
-    const uint32_t pr = PL ? lpairs[e] : vpairs[e];
-    if (pr != kNone) {
-      uint32_t pos = atomicAdd(&S.cursor[pr >> 16], 1u);
-      S.nb[pos] = (uint16_t)(pr & 0xFFFFu);
+  /* B2: scatter b into row order (overwrites the list1 bins, all read above);
+   * kU rows and bins are loaded per thread before the first atomic, so the
+   * global loads of bin2 overlap */
+  for (uint32_t e0 = tid; e0 < J.N2; e0 += NT * kU) {
+    uint32_t a[kU];
+    uint16_t b[kU];
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      const uint32_t e = e0 + u * NT;
+      a[u] = kBinNone;
+      b[u] = 0;
+      if (e < J.N2) {
+        a[u] = PL ? lpairs[e] : vpairs[e];
+        b[u] = J.bin2[e];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      if (a[u] != kBinNone) {
+        const uint32_t pos = atomicAdd(&S.cursor[a[u]], 1u);
+        S.nb[pos] = b[u];
+      }
     }
   }
   __syncthreads();
@@ -580,7 +626,13 @@ struct WaveStats {
   uint32_t cheap;                  /* cells passing the first-term test of the reverse sweep (per lane) */
 };
 
-__device__ __forceinline__ uint64_t cycles() { return clock64(); }
+__device__ __forceinline__ uint64_t cycles() {
+#if DTO_STATS
+  return clock64();
+#else
+  return 0;
+#endif
+}
 
 /* dto_tail_exact for one cell with the whole wave (wave-uniform arguments,
  * result in every lane): lane l evaluates term k + 64 b + l of block b, so a
@@ -638,7 +690,7 @@ __device__ __forceinline__ void flush_exact(const DevJob &J, const Lds &S, uint4
                             WaveStats &st) {
   const uint32_t lane = lane_id();
   wave_sync();
-  BestKey lb = {__builtin_inf(), 0u, kNone};
+  BestKey lb = no_best();
   uint32_t cells = 0;
   static_assert(kCand3Cap <= 64, "one lane per listed cell");
   const uint32_t nex = (uint32_t)__popcll(__ballot(lane < n3 && __uint_as_float(c3[lane].z) <= thr));
@@ -651,10 +703,11 @@ __device__ __forceinline__ void flush_exact(const DevJob &J, const Lds &S, uint4
       uint32_t nt = 0;
       const double p = wave_tail_exact(J, S, J.K[i], S.coln[j], k, J.D[j], &nt);
       if (lane == 0) {
+        /* every lane holds p: one lane enters the cell, so it counts once */
         st.terms += nt;
         ++cells;
+        take_better(lb, p, k, cd.x);
       }
-      take_better(lb, p, k, cd.x);
     }
   } else {
     for (uint32_t c = lane; c < n3; c += 64) {
@@ -668,10 +721,12 @@ __device__ __forceinline__ void flush_exact(const DevJob &J, const Lds &S, uint4
       take_better(lb, p, k, cd.x);
     }
   }
-  for (int off = 32; off >= 1; off >>= 1) cells += __shfl_xor(cells, off);
-  st.cells += cells;
+  if (DTO_STATS) {
+    for (int off = 32; off >= 1; off >>= 1) cells += __shfl_xor(cells, off);
+    st.cells += cells;
+  }
   lb = wave_best(lb);
-  take_better(xb, lb.p, lb.k, lb.ij);
+  merge_best(xb, lb);
   n3 = 0;
   wave_sync();
 }
@@ -936,24 +991,40 @@ __device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, cons
   __builtin_amdgcn_sched_barrier(0);
 }
 
-/* The p == 0 cell order (optimize_main.rs:93-116 at equal p): larger k, then
- * smaller (i, j). */
+/* The best of a class of cells that share one p (p == 0, or p == 1), in
+ * optimize's order at equal p (optimize_main.rs:93-116): larger k, then
+ * smaller (i, j); n counts the class's cells, nk those at the best k. */
 struct ZeroKey {
-  uint32_t k, ij;
+  uint32_t k, ij, n, nk;
 };
-__device__ __forceinline__ void take_zero(ZeroKey &z, uint32_t k, uint32_t ij) {
-  if (z.ij == kNone || k > z.k || (k == z.k && ij < z.ij)) {
-    z.k = k;
-    z.ij = ij;
+__device__ __forceinline__ void merge_zero(ZeroKey &a, const ZeroKey &b) {
+  if (b.ij == kNone) return;
+  if (a.ij == kNone) {
+    a = b;
+    return;
   }
+  const uint32_t n = a.n + b.n;
+  if (b.k > a.k) {
+    a = b;
+  } else if (b.k == a.k) {
+    a.nk += b.nk;
+    a.ij = b.ij < a.ij ? b.ij : a.ij;
+  }
+  a.n = n;
 }
+__device__ __forceinline__ void take_zero(ZeroKey &z, uint32_t k, uint32_t ij) { merge_zero(z, ZeroKey{k, ij, 1u, 1u}); }
 __device__ __forceinline__ ZeroKey wave_zero(ZeroKey z) {
   for (int off = 32; off >= 1; off >>= 1) {
-    const uint32_t k = __shfl_xor(z.k, off), ij = __shfl_xor(z.ij, off);
-    if (ij != kNone) take_zero(z, k, ij);
+    ZeroKey o;
+    o.k = __shfl_xor(z.k, off);
+    o.ij = __shfl_xor(z.ij, off);
+    o.n = __shfl_xor(z.n, off);
+    o.nk = __shfl_xor(z.nk, off);
+    merge_zero(z, o);
   }
   return z;
 }
+__device__ __forceinline__ BestKey class_key(double p, const ZeroKey &z) { return BestKey{p, z.k, z.ij, z.n, z.nk}; }
 
 /* Exact best among the p == 1 cells (k <= max(0, K+n-N)): larger k, then
  * smaller (i, j).  Only needed when no cell has p < 1; clears the wave's
@@ -964,7 +1035,7 @@ __device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout 
   const uint32_t w = wave_id(), lane = lane_id();
   const uint32_t T1 = J.T1, T2 = J.T2, T2w = J.T2w, N = J.N;
   const uint32_t c2 = (T2w + 63) / 64, wbeg = lane * c2;
-  ZeroKey o = {0u, kNone};
+  ZeroKey o = {0u, kNone, 0u, 0u};
   for (uint32_t t = lane; t < Lo.T2w_pad; t += 64) C[t] = 0u;
   wave_sync();
   uint32_t added = 0;
@@ -1005,13 +1076,26 @@ struct NextTask {
 template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS>
 __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
                                  const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
-                                 WaveStats &st, uint32_t *gpairs, const NextTask *nt = nullptr) {
+                                 WaveStats &st, uint16_t *gpairs, const NextTask *nt = nullptr) {
   constexpr uint32_t W = NT / 64;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   uint64_t tc0 = cycles();
   build_rows<NT, EXPLICIT, HLDS, PL>(J, Lo, S, i1, i2, s1, s2, err, gpairs);
   uint64_t tc1 = cycles();
   st.cyc[0] += tc1 - tc0;
+  /* wave 0 claims the workgroup's next task once the build is done: by then
+   * every workgroup of the launch has started, so none takes a second task
+   * while another still has none.  The claim does not depend on the grid's
+   * contents, and the slot is read after this task's last barrier, so every
+   * task moves the workgroup on to a new one. */
+  uint32_t pnext = kNone;
+  if (nt && w == 0u) {
+    if (lane == 0) {
+      pnext = atomicAdd(nt->counter, 1u);
+      *nt->slot = pnext;
+    }
+    pnext = (uint32_t)__builtin_amdgcn_readfirstlane((int)pnext);
+  }
 
   const uint32_t T1 = J.T1, T2 = J.T2, T2w = J.T2w;
   uint32_t *C = S.cw + w * Lo.T2w_pad;
@@ -1020,7 +1104,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   const char *Hb = HLDS ? (const char *)S.h : (const char *)J.H32;
 
   if (PLAIN) {
-    BestKey xb = {__builtin_inf(), 0u, kNone};
+    BestKey xb = no_best();
     uint32_t added = 0;
     for (uint32_t i = w; i < T1; i += W) {
       rows_update(S, C, added, i + 1, true);
@@ -1044,11 +1128,10 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     }
     xb = wave_best(xb);
     if (lane == 0) S.redB[2 * W + w] = xb;
-    if (nt && tid == 0) *nt->slot = atomicAdd(nt->counter, 1u);
     __syncthreads();
-    BestKey b = {__builtin_inf(), 0u, kNone};
+    BestKey b = no_best();
     if (tid == 0)
-      for (uint32_t v = 0; v < W; ++v) take_better(b, S.redB[2 * W + v].p, S.redB[2 * W + v].k, S.redB[2 * W + v].ij);
+      for (uint32_t v = 0; v < W; ++v) merge_best(b, S.redB[2 * W + v]);
     __syncthreads();
     return b;
   }
@@ -1069,8 +1152,8 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   const float margin = J.margin;
   const float uthr = -745.2f - margin; /* below: the first term may underflow */
 
-  ZeroKey zb = {0u, kNone};          /* p == 0 cells */
-  BestKey xb = {__builtin_inf(), 0u, kNone}; /* exactly evaluated cells */
+  ZeroKey zb = {0u, kNone, 0u, 0u}; /* p == 0 cells */
+  BestKey xb = no_best();           /* exactly evaluated cells */
   /* this lane's cell with the smallest screen value: its tail bracket, taken
    * after the sweep, bounds the minimum log p from above */
   float bx = __builtin_inff();
@@ -1089,20 +1172,11 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   uint32_t Kl = 0u;
   float Al = 0.f;
   uint32_t pfacc = 0u; /* keeps the prefetch loads */
-  uint32_t pnext = kNone; /* wave 0: the next task */
   for (uint32_t i = w, q = 0; i < T1; i += W, ++q) {
     if ((q & 63u) == 0u) {
       const uint32_t ir = i + lane * W;
       Kl = ir < T1 ? J.K[ir] : 0u;
       Al = ir < T1 ? J.A32[ir] : 0.f;
-      if (q == 0u && nt && w == 0u) {
-        /* wave 0 claims the workgroup's next task here: every workgroup of
-         * the launch has started by now, so none takes a second task while
-         * another still has none */
-        if (lane == 0) pnext = atomicAdd(nt->counter, 1u);
-        pnext = (uint32_t)__builtin_amdgcn_readfirstlane((int)pnext);
-        if (lane == 0) *nt->slot = pnext;
-      }
     }
     add_bins(S, C, pe0, pe1, pb);
     added = i + 1;
@@ -1207,7 +1281,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   st.cyc[1] += tc2 - tc1;
   if (lane == 0) {
     S.redU[w] = Urun;
-    S.redB[w] = BestKey{0.0, zb.k, zb.ij};
+    S.redB[w] = class_key(0.0, zb);
   }
   /* no barrier here: each wave goes on to its own rows' reverse sweep as
    * soon as its forward sweep is done, starting from its own bound, and
@@ -1352,24 +1426,31 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   }
   if (need1) {
     const ZeroKey o = class1_best<NT>(J, Lo, S, C);
-    if (lane == 0) S.redB[W + w] = BestKey{1.0, o.k, o.ij};
+    if (lane == 0) S.redB[W + w] = class_key(1.0, o);
   } else if (lane == 0) {
-    S.redB[W + w] = BestKey{1.0, 0u, kNone};
+    S.redB[W + w] = no_best();
   }
   __syncthreads();
-  BestKey b = {__builtin_inf(), 0u, kNone};
+  /* the three classes hold disjoint cells: p == 0 cells flagged by the
+   * forward sweep (never evaluated), p == 1 cells (k <= max(0, K + n - N)),
+   * and the evaluated sum-class cells; the tie counts add up across them */
+  BestKey b = no_best();
   if (tid == 0) {
-    for (uint32_t v = 0; v < 3 * W; ++v) take_better(b, S.redB[v].p, S.redB[v].k, S.redB[v].ij);
-    float U0 = __builtin_inff(); /* the forward sweeps' bound */
-    for (uint32_t v = 0; v < W; ++v) U0 = fminf(U0, S.redU[v]);
-    if (b.p > 0.0 && b.p < 1.0 && U0 < 1e30f) st.gap += (uint32_t)fmaxf(0.f, (U0 - (float)log(b.p)) * 1000.f);
+    for (uint32_t v = 0; v < 3 * W; ++v) merge_best(b, S.redB[v]);
+    if (DTO_STATS) {
+      float U0 = __builtin_inff(); /* the forward sweeps' bound */
+      for (uint32_t v = 0; v < W; ++v) U0 = fminf(U0, S.redU[v]);
+      if (b.p > 0.0 && b.p < 1.0 && U0 < 1e30f) st.gap += (uint32_t)fmaxf(0.f, (U0 - (float)log(b.p)) * 1000.f);
+    }
   }
   __syncthreads();
   return b;
 }
 
-/* Adds one wave's work counters to the job's counters (one atomic each per wave). */
+/* Adds one wave's work counters to the job's counters (one atomic each per
+ * wave); nothing without DTO_STATS. */
 __device__ __forceinline__ void flush_stats(const DevJob &J, const WaveStats &st) {
+  if (!DTO_STATS) return;
   uint32_t t = st.terms, ch = st.cheap;
   for (int off = 32; off >= 1; off >>= 1) {
     t += __shfl_xor(t, off);
@@ -1411,14 +1492,14 @@ __device__ __forceinline__ void write_record(const DevJob &J, const BestKey &b, 
   r.permuted = permuted;
   r.idx1 = i;
   r.idx2 = j;
-  r.reserved = 0;
+  r.ties = (b.np < 0xFFFFu ? b.np : 0xFFFFu) | ((b.npk < 0xFFFFu ? b.npk : 0xFFFFu) << 16);
   *out = r;
 }
 
 /* Permutations [perm_base, perm_base + n_perm) of one job.  EXPLICIT: idx1 /
  * idx2 hold n_perm permutations back to back (NULL: identity); otherwise the
  * Philox shuffle keyed by (seed, job_id, permutation id, list).  ws: per
- * workgroup pair staging (N2 words each) when !PL.  Workgroups take
+ * workgroup pair staging (N2 u16 each) when !PL.  Workgroups take
  * permutations from a device counter (zeroed before the launch), so that
  * permutations of uneven cost spread evenly; every workgroup exits once the
  * counter passes n_perm. */
@@ -1427,7 +1508,7 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
                                                  const uint32_t *__restrict__ idx2, uint64_t n_perm,
                                                  uint64_t perm_base, uint64_t seed, uint32_t job_id,
                                                  int32_t permuted, dto_record *__restrict__ out,
-                                                 int *__restrict__ err, uint32_t *__restrict__ ws,
+                                                 int *__restrict__ err, uint16_t *__restrict__ ws,
                                                  uint32_t *__restrict__ counter) {
   /* no static LDS here: the dynamic region then starts at address 0 */
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1436,7 +1517,7 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
   const Lds S = lds_view(smem, Lo, W);
   load_job_tables<NT, HLDS>(J, S);
   __syncthreads();
-  uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * J.N2;
+  uint16_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * J.N2;
   WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
   /* the first task; each task claims the next one (optimize_perm) */
   if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
@@ -1456,7 +1537,10 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
       s2 = shuffle_make(seed, job_id, (uint32_t)pid, 2u, J.N2);
     }
     BestKey b = optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS>(J, Lo, S, i1, i2, s1, s2, err, st, gpairs, &nt);
-    const uint64_t pn = s_next; /* published by optimize_perm's last barrier */
+    /* a counter value claimed during this task (optimize_perm claims one
+     * every time, after the build): published by its last barrier, and new,
+     * so the loop always moves on */
+    const uint64_t pn = s_next;
     if (threadIdx.x == 0) write_record(J, b, permuted, out + pp, err);
     pp = pn;
   }
@@ -1504,7 +1588,7 @@ static __global__ void k_reset_counter(uint32_t *counter, int *err) {
 template <int NT, bool HLDS, bool PL, bool REGCOLS>
 __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs, const WorkItem *__restrict__ items,
                                               uint32_t n_items, uint64_t seed, uint32_t *__restrict__ counter,
-                                              int *__restrict__ err, uint32_t *__restrict__ ws, uint32_t ws_stride,
+                                              int *__restrict__ err, uint16_t *__restrict__ ws, uint32_t ws_stride,
                                               uint32_t item_slot) {
   /* no static LDS here: the dynamic region then starts at address 0.  The
    * work item slot is the misc slot of the batch's largest layout, which
@@ -1512,7 +1596,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs,
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int W = NT / 64;
   uint32_t &s_item = *(uint32_t *)(smem + item_slot);
-  uint32_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * ws_stride;
+  uint16_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * ws_stride;
   uint32_t cur = kNone;
   WaveStats st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
   const DevJob *last = nullptr;
@@ -1556,7 +1640,7 @@ template <int NT, bool PL>
 __global__ __launch_bounds__(NT) void k_grid_counts(DevJob J, LdsLayout Lo, const uint32_t *__restrict__ idx1,
                                                     const uint32_t *__restrict__ idx2,
                                                     uint32_t *__restrict__ kout, int *__restrict__ err,
-                                                    uint32_t *__restrict__ ws) {
+                                                    uint16_t *__restrict__ ws) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int W = NT / 64;
   const Lds S = lds_view(smem, Lo, W);

@@ -3,6 +3,7 @@ compute_population_size and their result types, on the GPU through the C ABI.
 """
 import ctypes
 import os
+import sys
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -64,8 +65,11 @@ class Job:
 
     # ---- construction -------------------------------------------------------
     @classmethod
-    def from_lists(cls, ids1, ranks1, ids2, ranks2, background=None, drop_outside_background=False):
-        """RankedFeatureList::from for both lists + compute_population_size."""
+    def from_lists(cls, ids1, ranks1, ids2, ranks2, background=None, drop_outside_background=False,
+                   defer_upload=False):
+        """RankedFeatureList::from for both lists + compute_population_size.
+        defer_upload: keep the tables on the host until a server admits the
+        job (or another run first needs them)."""
         L = _lib.lib()
         a, b = _lib.cstr_array(list(ids1)), _lib.cstr_array(list(ids2))
         r1 = np.ascontiguousarray(ranks1, dtype=np.uint32)
@@ -73,8 +77,24 @@ class Job:
         bg = None if background is None else _lib.cstr_array(list(background))
         h = ctypes.c_void_p()
         flags = _lib.DTO_DROP_OUTSIDE_BACKGROUND if drop_outside_background else 0
+        if defer_upload:
+            flags |= _lib.DTO_DEFER_UPLOAD
         _lib.check(L.dto_job_create(a, _lib.u32p(r1), len(r1), b, _lib.u32p(r2), len(r2), bg,
                                     0 if background is None else len(background), flags, ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_c_arrays(cls, ids1, ranks1, ids2, ranks2, background=None, n_background=0,
+                      drop_outside_background=False, defer_upload=False):
+        """from_lists on id arrays already in C form (_lib.cstr_array) and
+        uint32 rank arrays, for building many jobs from the same lists."""
+        h = ctypes.c_void_p()
+        flags = _lib.DTO_DROP_OUTSIDE_BACKGROUND if drop_outside_background else 0
+        if defer_upload:
+            flags |= _lib.DTO_DEFER_UPLOAD
+        _lib.check(_lib.lib().dto_job_create(ids1, _lib.u32p(ranks1), len(ranks1), ids2, _lib.u32p(ranks2),
+                                             len(ranks2), background, n_background if background is not None else 0,
+                                             flags, ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -165,6 +185,28 @@ class Job:
                                              d_idx1, d_idx2, _lib.MODES[mode], d_out, stream))
 
 
+def tie_warnings(rec):
+    """The stderr lines optimize prints for a record whose minimum p is shared
+    (src/dto/optimize_main.rs:86-108); rec is a dto_record (numpy or ctypes)."""
+    ties = int(rec["ties"]) if not hasattr(rec, "ties") else int(rec.ties)
+    n_p, n_pk = ties & 0xFFFF, ties >> 16
+    out = []
+    if n_p > 1:
+        p = float(rec["pvalue"]) if not hasattr(rec, "pvalue") else float(rec.pvalue)
+        k = int(rec["intersection_size"]) if not hasattr(rec, "intersection_size") else int(rec.intersection_size)
+        out.append("Multiple results with the same minimum p-value (%.15f). Choosing the result with the largest "
+                   "intersection size." % p)
+        if n_pk > 1:
+            out.append("Multiple results with the same maximum intersection size (%d). Choosing an arbitrary result "
+                       "based on the order of thresholds." % k)
+    return out
+
+
+def _warn_ties(rec):
+    for line in tie_warnings(rec):
+        print(line, file=sys.stderr)
+
+
 def _job_for(l1, l2, population_size):
     return Job.with_population(l1.ids(), l1.ranks(), l2.ids(), l2.ranks(), population_size)
 
@@ -233,6 +275,7 @@ def optimize(ranked_feature_list1, ranked_feature_list2, permute, population_siz
         r = job.optimize(a, b, mode)
     finally:
         job.close()
+    _warn_ties(r)
     rec = OptimizationResultRecord.from_c(r)
     rec.permuted = bool(permute)
     return OptimizationResult(best=rec)

@@ -8,7 +8,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .dto import Job, OptimizationResult, OptimizationResultRecord
+from .dto import Job, OptimizationResult, OptimizationResultRecord, _warn_ties
 from .stat_operations import result_dict
 
 
@@ -32,9 +32,14 @@ def run_single_node(tasks, ranked_feature_list1, ranked_feature_list2, populatio
         res, perm = job.run(n_perm, seed=seed, mode=mode)
     finally:
         job.close()
+    # each task's optimize warns about its own ties (optimize_main.rs:86-108)
+    n_unperm = sum(1 for t in tasks if not t.permute)
+    for _ in range(n_unperm):
+        _warn_ties(res.unpermuted)
+    for r in perm:
+        _warn_ties(r)
     u = res.unpermuted
     out = []
-    n_unperm = sum(1 for t in tasks if not t.permute)
     for _ in range(n_unperm):
         out.append(OptimizationResult(best=OptimizationResultRecord(
             u.rank1, u.rank2, u.set1_len, u.set2_len, u.population_size, u.intersection_size, u.pvalue, False)))

@@ -26,9 +26,8 @@ def _min_ties(n, frac, rng):
     if k and n > 1:
         merged = np.zeros(n, dtype=bool)
         merged[rng.choice(np.arange(1, n), size=min(k, n - 1), replace=False)] = True
-        for p in range(1, n):
-            if merged[p]:
-                ranks[p] = ranks[p - 1]
+        # a merged position takes the rank of the last unmerged one before it
+        ranks = np.maximum.accumulate(np.where(merged, 0, ranks))
     return ranks
 
 

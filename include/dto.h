@@ -41,18 +41,26 @@ extern "C" {
 /* job flags */
 #define DTO_DROP_OUTSIDE_BACKGROUND 0x1u /* drop list features absent from the
                                             background instead of failing */
+#define DTO_DEFER_UPLOAD 0x2u /* build the job's tables on the host only: a
+                                 server copies them into its HBM pool when it
+                                 admits the job; any other run uploads them
+                                 on first use */
 
 /* run modes */
 #define DTO_MODE_SCREENED 0 /* FP32 log-space screen + exact FP64 recheck */
 #define DTO_MODE_PLAIN 1    /* exact FP64 p-value on every grid cell */
 
 /* Limits of the device kernels (checked at job creation). */
-#define DTO_MAX_LIST_LEN 32768u
+#define DTO_MAX_LIST_LEN 65535u /* and the LDS check: lists up to about 61,000 entries fit */
 #define DTO_MAX_THRESHOLDS 4096u
 
 /* One optimisation record: OptimizationResultRecord without the debug
  * feature sets (src/dto/results_objects.rs:21-32). rank1/rank2 are threshold
- * values; idx1/idx2 their positions in the threshold lists. */
+ * values; idx1/idx2 their positions in the threshold lists.  ties (records
+ * from optimize only; 0 in grid records): bits 0-15 count the grid cells at
+ * the minimum p, bits 16-31 those of them at the largest intersection, each
+ * saturated at 0xFFFF; optimize warns on stderr when either is above 1
+ * (src/dto/optimize_main.rs:86-108, see DTO_TIES_AT_MIN_P). */
 typedef struct {
   uint64_t rank1, rank2;
   uint64_t set1_len, set2_len;
@@ -61,8 +69,10 @@ typedef struct {
   double pvalue;
   int32_t permuted;
   uint32_t idx1, idx2;
-  uint32_t reserved;
+  uint32_t ties;
 } dto_record;
+#define DTO_TIES_AT_MIN_P(r) ((r)->ties & 0xFFFFu)
+#define DTO_TIES_AT_MAX_K(r) ((r)->ties >> 16)
 
 /* Result of a whole DTO run: the fields of the JSON object built by
  * empirical_pvalue (src/stat_operations/empirical_pvalue.rs:136-186). */

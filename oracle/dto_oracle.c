@@ -313,7 +313,7 @@ static int or_grid_rows(const or_list *l1, const or_list *l2, const uint64_t *id
         r->rank1 = l1->thr[i]; r->rank2 = l2->thr[j];
         r->set1_len = n1; r->set2_len = g2n[j];
         r->population_size = N; r->intersection_size = k;
-        r->pvalue = p; r->permuted = permuted; r->i = i; r->j = j;
+        r->pvalue = p; r->permuted = permuted; r->i = i; r->j = j; r->ties = 0;
       } else {
         /* keep the work observable for the timing sample */
         static __thread volatile double sink;
@@ -345,11 +345,16 @@ static int or_select_best(const or_record *r, uint64_t n, or_record *best) {
       found = 1;
     }
   const or_record *b = NULL;
+  uint64_t n_p = 0, n_pk = 0; /* best_results.len() at optimize_main.rs:86 and :102 */
   for (uint64_t c = 0; c < n; c++) {
-    if (r[c].pvalue != minp || r[c].intersection_size != maxk) continue;
+    if (r[c].pvalue != minp) continue;
+    n_p++;
+    if (r[c].intersection_size != maxk) continue;
+    n_pk++;
     if (!b || r[c].rank1 < b->rank1 || (r[c].rank1 == b->rank1 && r[c].rank2 < b->rank2)) b = &r[c];
   }
   *best = *b;
+  best->ties = (uint32_t)(n_p < 0xFFFF ? n_p : 0xFFFF) | ((uint32_t)(n_pk < 0xFFFF ? n_pk : 0xFFFF) << 16);
   return 0;
 }
 
@@ -496,7 +501,7 @@ int or_fast_optimize(const or_list *l1, const or_list *l2, const uint64_t *idx1,
         c->rank1 = l1->thr[i]; c->rank2 = l2->thr[j];
         c->set1_len = K[i]; c->set2_len = n2[j]; c->population_size = N;
         c->intersection_size = (uint64_t)k[(uint64_t)i * T2 + j];
-        c->pvalue = p[(uint64_t)i * T2 + j]; c->permuted = permuted; c->i = i; c->j = j;
+        c->pvalue = p[(uint64_t)i * T2 + j]; c->permuted = permuted; c->i = i; c->j = j; c->ties = 0;
       }
     rc = or_select_best(r, n, best);
     free(r); free(K); free(n2);

@@ -84,6 +84,11 @@ typedef struct {
   double pvalue;
   int32_t permuted;
   uint32_t i, j; /* threshold indices */
+  /* optimize only (0 in grid records): how many cells share the minimum p
+   * (low 16 bits) and how many of those share the largest intersection
+   * (high 16 bits), each saturated at 0xFFFF; optimize_main.rs:86-108 warns
+   * when either is above 1 */
+  uint32_t ties;
 } or_record;
 
 /* process_threshold_pairs with the reference's per-cell hash-set

@@ -28,7 +28,12 @@ class Record(ctypes.Structure):
         ("permuted", ctypes.c_int32),
         ("i", ctypes.c_uint32),
         ("j", ctypes.c_uint32),
+        ("ties", ctypes.c_uint32),
     ]
+
+    def tie_counts(self):
+        """(#cells at the minimum p, #of those at the largest intersection), saturated at 0xFFFF."""
+        return (self.ties & 0xFFFF, self.ties >> 16)
 
     def as_tuple(self):
         return (self.rank1, self.rank2, self.set1_len, self.set2_len,
@@ -242,7 +247,7 @@ def optimize_many(l1, l2, N, idx1, idx2, fast=True, threads=None):
     P = a.shape[0]
     out = (Record * max(P, 1))()
     rc = lib().or_optimize_many(l1.h, l2.h, _u64p(a), _u64p(b), P, int(N), 1 if fast else 0,
-                                int(threads or os.cpu_count() or 1), out)
+                                int(threads or min(16, os.cpu_count() or 1)), out)
     if rc:
         raise ValueError("oracle optimize failed (%d)" % rc)
     return [out[p] for p in range(P)]

@@ -31,11 +31,16 @@ def ten_gene_lists():
 
 
 def rec_tuple(r):
-    """(rank1, rank2, set1_len, set2_len, N, k, p) of a product record (numpy) or oracle record."""
+    """(rank1, rank2, set1_len, set2_len, N, k, p, ties) of a product record
+    (numpy or ctypes) or an oracle record; ties packs the counts behind
+    optimize's two tie warnings (0 in grid records)."""
     if hasattr(r, "as_tuple"):
-        return r.as_tuple()
+        return r.as_tuple() + (int(r.ties),)
+    if hasattr(r, "_fields_"):
+        return (int(r.rank1), int(r.rank2), int(r.set1_len), int(r.set2_len), int(r.population_size),
+                int(r.intersection_size), float(r.pvalue), int(r.ties))
     return (int(r["rank1"]), int(r["rank2"]), int(r["set1_len"]), int(r["set2_len"]), int(r["population_size"]),
-            int(r["intersection_size"]), float(r["pvalue"]))
+            int(r["intersection_size"]), float(r["pvalue"]), int(r["ties"]))
 
 
 def random_lists(rng, n, overlap_frac=1.0, tie_frac=0.1, dup=False, extra=0, max_rank=None):

@@ -166,3 +166,59 @@ def test_batch_autotune_keeps_results(dto, gpu):
         assert rec_tuple_obj(x.unpermuted) == rec_tuple_obj(y.unpermuted)
         assert (x.empirical_pvalue, x.fdr, x.n_le) == (y.empirical_pvalue, y.fdr, y.n_le)
     b.close()
+
+
+@pytest.mark.parametrize("python", [False, True])
+def test_cli_tie_warnings(dto, O, gpu, tmp_path, python):
+    """optimize_main.rs:86-108: a task whose minimum p is shared prints the
+    first warning (with {:.15} of the p-value), and the second when the
+    largest intersection is shared too; one pair of lines per such task, in
+    task order here.  The counts come from the kernel and equal the oracle's."""
+    # list-2 duplicates of one list-1 id give several p == 0 cells (k > K)
+    ids1 = ["a", "b", "c", "d", "e", "f"]
+    ids2 = ["a", "a", "a", "b", "c", "x"]
+    bg = ids1 + ["x", "y"]
+    p1 = tmp_path / "l1.csv"
+    p2 = tmp_path / "l2.csv"
+    pb = tmp_path / "bg.txt"
+    p1.write_text("".join("%s,%d\n" % (g, r + 1) for r, g in enumerate(ids1)))
+    p2.write_text("".join("%s,%d\n" % (g, r + 1) for r, g in enumerate(ids2)))
+    pb.write_text("\n".join(bg) + "\n")
+    P, seed = 6, 123
+    r = _cli(["-1", str(p1), "-2", str(p2), "-b", str(pb), "-p", str(P), "--seed", str(seed)], python)
+    assert r.returncode == 0, r.stderr
+    l1 = O.OracleList(ids1, list(range(1, 7)))
+    l2 = O.OracleList(ids2, list(range(1, 7)))
+    N = len(bg)
+    recs = [O.optimize(l1, l2, N)]
+    i1 = np.stack([dto.philox_permutation(seed, 0, p, 1, 6) for p in range(P)])
+    i2 = np.stack([dto.philox_permutation(seed, 0, p, 2, 6) for p in range(P)])
+    recs += O.optimize_many(l1, l2, N, i1, i2, fast=False)
+    want = []
+    for o in recs:
+        n_p, n_pk = o.tie_counts()
+        if n_p > 1:
+            want.append("Multiple results with the same minimum p-value (%.15f). Choosing the result with the "
+                        "largest intersection size." % o.pvalue)
+            if n_pk > 1:
+                want.append("Multiple results with the same maximum intersection size (%d). Choosing an arbitrary "
+                            "result based on the order of thresholds." % o.intersection_size)
+    assert recs[0].tie_counts()[0] > 1 and any("(0.000000000000000)" in w for w in want)
+    got = [ln for ln in r.stderr.splitlines() if ln.startswith("Multiple results")]
+    assert got == want
+
+
+def test_optimize_prints_tie_warnings(dto, O, gpu, capsys):
+    """The crate's optimize prints the warnings itself; so does the mirror."""
+    l1 = dto.RankedFeatureList.from_(dto.FeatureList(["a", "b", "c"]), [1, 2, 3])
+    l2 = dto.RankedFeatureList.from_(dto.FeatureList(["a", "a", "a", "b"]), [1, 2, 3, 4])
+    best = dto.optimize(l1, l2, False, 5).best
+    o = O.optimize(O.OracleList(["a", "b", "c"], [1, 2, 3]), O.OracleList(["a", "a", "a", "b"], [1, 2, 3, 4]), 5)
+    assert (best.rank1, best.rank2, best.intersection_size, best.pvalue) == (o.rank1, o.rank2, o.intersection_size,
+                                                                             o.pvalue)
+    err = capsys.readouterr().err.splitlines()
+    n_p, n_pk = o.tie_counts()
+    assert n_p > 1
+    assert err[0] == ("Multiple results with the same minimum p-value (0.000000000000000). Choosing the result with "
+                      "the largest intersection size.")
+    assert len(err) == (2 if n_pk > 1 else 1)

@@ -103,7 +103,8 @@ def test_optimize_known_answers(dto, mode, gpu):
     ids1, r1, ids2, r2 = readme_lists()
     job = dto.Job.from_lists(ids1, r1, ids2, r2)
     b = job.optimize(mode=mode)
-    assert rec_tuple(b) == (24, 13, 24, 13, 30, 12, 0.15632183908046102)
+    # README.md:174-186: a unique minimum (one cell at its p, hence no tie warning)
+    assert rec_tuple(b) == (24, 13, 24, 13, 30, 12, 0.15632183908046102, 0x10001)
 
 
 @pytest.mark.parametrize("mode", ["plain", "screened"])
@@ -157,10 +158,10 @@ def test_run_plain_6k_explicit(dto, O, mode, gpu):
     N = O.population_size(l1, l2)
     ub = O.optimize(l1, l2, N, fast=True)
     u = res.unpermuted
-    assert (u.rank1, u.rank2, u.set1_len, u.set2_len, u.population_size, u.intersection_size, u.pvalue) == ub.as_tuple()
+    assert rec_tuple(u) == rec_tuple(ub)
     ob = O.optimize_many(l1, l2, N, i1, i2, fast=True)
     for p in range(P):
-        assert rec_tuple(perm[p]) == ob[p].as_tuple(), p
+        assert rec_tuple(perm[p]) == rec_tuple(ob[p]), p
     le = sum(1 for b in ob if b.pvalue <= ub.pvalue)
     assert res.empirical_pvalue == le / P
     assert res.fdr == O.fdr(ub.set1_len, ub.set2_len, ub.intersection_size, N)
@@ -176,7 +177,7 @@ def test_run_philox_matches_oracle(dto, O, gpu):
     l1, l2 = _oracle_pair(O, ids1, r1, ids2, r2)
     ob = O.optimize_many(l1, l2, 30, i1, i2, fast=False)
     for p in range(P):
-        assert rec_tuple(perm[p]) == ob[p].as_tuple(), p
+        assert rec_tuple(perm[p]) == rec_tuple(ob[p]), p
 
 
 def test_genome_20k_vs_oracle(dto, O, gpu):
@@ -194,10 +195,36 @@ def test_genome_20k_vs_oracle(dto, O, gpu):
     l2 = O.OracleList([a for a, _ in f2], [b for _, b in f2])
     ub = O.optimize(l1, l2, 20000, fast=True)
     u = res.unpermuted
-    assert (u.rank1, u.rank2, u.set1_len, u.set2_len, u.population_size, u.intersection_size, u.pvalue) == ub.as_tuple()
+    assert rec_tuple(u) == rec_tuple(ub)
     ob = O.optimize_many(l1, l2, 20000, i1, i2, fast=True)
     for p in range(P):
-        assert rec_tuple(perm[p]) == ob[p].as_tuple(), p
+        assert rec_tuple(perm[p]) == rec_tuple(ob[p]), p
+
+
+def test_genome_20k_fixture_many_permutations(dto, gpu):
+    """The genome-scale job on the 256 explicit permutations whose oracle
+    records are committed (tests/golden/make_genome_fixture.py): every
+    record, p-value bits and tie counts, in both modes."""
+    import json
+    import os
+    from dual_threshold_optimization_amd import workloads as W
+    from helpers import GOLDEN
+    with open(os.path.join(GOLDEN, "genome_20k_perms.json")) as f:
+        fx = json.load(f)
+    ids1, r1, ids2, r2, bg = W.genome_20k()
+    job = dto.Job.from_lists(ids1, r1, ids2, r2, background=bg, drop_outside_background=True)
+    P = len(fx["permuted"])
+    i1, i2 = W.explicit_permutations(P, job.len1, job.len2, 47)
+
+    def dec(d):
+        return tuple(d[:6]) + (float.fromhex(d[6]), d[7])
+    for mode in ("screened", "plain"):
+        res, perm = job.run(P, idx1=i1, idx2=i2, mode=mode)
+        assert rec_tuple(res.unpermuted) == dec(fx["unpermuted"]), mode
+        bad = [p for p in range(P) if rec_tuple(perm[p]) != dec(fx["permuted"][p])]
+        assert not bad, (mode, bad[:5], [rec_tuple(perm[p]) for p in bad[:2]], [fx["permuted"][p] for p in bad[:2]])
+        le = sum(1 for d in fx["permuted"] if float.fromhex(d[6]) <= float.fromhex(fx["unpermuted"][6]))
+        assert res.n_le == le and res.empirical_pvalue == le / P
 
 
 def test_screened_equals_plain_genome_20k(dto, gpu):
@@ -255,10 +282,9 @@ def test_dense_thresholds_vs_oracle(dto, O, gpu):
         for mode in ("screened", "plain"):
             res, perm = job.run(P, idx1=i1, idx2=i2, mode=mode)
             u = res.unpermuted
-            assert (u.rank1, u.rank2, u.set1_len, u.set2_len, u.population_size, u.intersection_size,
-                    u.pvalue) == ub.as_tuple(), (n, mode)
+            assert rec_tuple(u) == rec_tuple(ub), (n, mode)
             for p in range(P):
-                assert rec_tuple(perm[p]) == ob[p].as_tuple(), (n, mode, p)
+                assert rec_tuple(perm[p]) == rec_tuple(ob[p]), (n, mode, p)
 
 
 def test_dense_2k_genome_20k_fixture(dto, gpu):
@@ -280,12 +306,11 @@ def test_dense_2k_genome_20k_fixture(dto, gpu):
 
     def dec(d):
         return (d["rank1"], d["rank2"], d["set1_len"], d["set2_len"], d["population_size"],
-                d["intersection_size"], float.fromhex(d["pvalue"]))
+                d["intersection_size"], float.fromhex(d["pvalue"]), d["ties"])
     for mode in ("screened", "plain"):
         res, perm = job.run(P, idx1=i1, idx2=i2, mode=mode)
         u = res.unpermuted
-        assert (u.rank1, u.rank2, u.set1_len, u.set2_len, u.population_size, u.intersection_size,
-                u.pvalue) == dec(fx["unpermuted"]), mode
+        assert rec_tuple(u) == dec(fx["unpermuted"]), mode
         for p in range(P):
             assert rec_tuple(perm[p]) == dec(fx["permuted"][p]), (mode, p)
     rs, ps = job.run(24, seed=31, mode="screened")
@@ -304,10 +329,10 @@ def test_batch_vs_oracle_and_graph_replay(dto, O, gpu):
     seed = 31
     b = dto.Batch(jobs, n_perm, seed=seed)
     b.launch()
-    first = [(r.unpermuted.rank1, r.unpermuted.rank2, r.unpermuted.pvalue, r.n_le) for r in b.results()]
+    first = [(rec_tuple(r.unpermuted), r.n_le) for r in b.results()]
     for _ in range(3):  # graph replays give the same answers
         b.launch()
-        again = [(r.unpermuted.rank1, r.unpermuted.rank2, r.unpermuted.pvalue, r.n_le) for r in b.results()]
+        again = [(rec_tuple(r.unpermuted), r.n_le) for r in b.results()]
         assert again == first
     for j, (ids1, r1, ids2, r2) in enumerate(lists):
         l1, l2 = _oracle_pair(O, ids1, r1, ids2, r2)
@@ -318,7 +343,7 @@ def test_batch_vs_oracle_and_graph_replay(dto, O, gpu):
         ub = O.optimize(l1, l2, N, fast=True)
         ob = O.optimize_many(l1, l2, N, i1, i2, fast=True)
         le = sum(1 for x in ob if x.pvalue <= ub.pvalue)
-        assert first[j] == (ub.rank1, ub.rank2, ub.pvalue, le), j
+        assert first[j] == (rec_tuple(ub), le), j
 
 
 def test_server_mix_equals_alone(dto, O, gpu):
@@ -348,6 +373,5 @@ def test_server_mix_equals_alone(dto, O, gpu):
         ob = O.optimize_many(l1, l2, N, i1, i2, fast=True)
         le = sum(1 for x in ob if x.pvalue <= ub.pvalue)
         r = got[rid]
-        assert (r.unpermuted.rank1, r.unpermuted.rank2, r.unpermuted.pvalue, r.n_le) == (ub.rank1, ub.rank2,
-                                                                                        ub.pvalue, le)
+        assert (rec_tuple(r.unpermuted), r.n_le) == (rec_tuple(ub), le)
     sv.close()

@@ -2,9 +2,9 @@
 point where a job's tables leave LDS.
 
 From N ~ 20.5k the screen table no longer fits in LDS next to the pair
-staging and the kernels read it from HBM; larger jobs also stage their
-pairs outside LDS.  These tests run every such launch shape against the
-oracle on the same explicit or Philox permutations (bit-exact records)."""
+staging and the kernels read it from HBM; from N ~ 31k the pair staging
+leaves LDS too.  These tests run every such launch shape against the oracle
+on the same explicit or Philox permutations (bit-exact records)."""
 import numpy as np
 import pytest
 
@@ -27,7 +27,7 @@ def _genome_like(W, n, seed):
     return W.planted_lists(n, int(0.03 * n), n // 10, (3 * n) // 40, 0.10, seed, extras=2000)
 
 
-@pytest.mark.parametrize("n", [24000, 30000])
+@pytest.mark.parametrize("n", [24000, 30000, 40000])
 def test_large_job_vs_oracle(dto, O, gpu, n):
     from dual_threshold_optimization_amd import workloads as W
     ids1, r1, ids2, r2, bg = _genome_like(W, n, 40 + n)
@@ -42,17 +42,16 @@ def test_large_job_vs_oracle(dto, O, gpu, n):
         print("large job N=%d mode=%s" % (n, mode), flush=True)
         res, perm = job.run(P, idx1=i1, idx2=i2, mode=mode)
         u = res.unpermuted
-        assert (u.rank1, u.rank2, u.set1_len, u.set2_len, u.population_size, u.intersection_size,
-                u.pvalue) == ub.as_tuple(), (n, mode)
+        assert rec_tuple(u) == rec_tuple(ub), (n, mode)
         for p in range(P):
-            assert rec_tuple(perm[p]) == ob[p].as_tuple(), (n, mode, p)
+            assert rec_tuple(perm[p]) == rec_tuple(ob[p]), (n, mode, p)
     # Philox shuffles of the same job: screened against the oracle on the drawn permutations
     res, perm = job.run(4, seed=3 + n)
     j1 = np.stack([dto.philox_permutation(3 + n, 0, p, 1, n) for p in range(4)])
     j2 = np.stack([dto.philox_permutation(3 + n, 0, p, 2, n) for p in range(4)])
     oph = O.optimize_many(l1, l2, n, j1, j2, fast=True)
     for p in range(4):
-        assert rec_tuple(perm[p]) == oph[p].as_tuple(), (n, "philox", p)
+        assert rec_tuple(perm[p]) == rec_tuple(oph[p]), (n, "philox", p)
 
 
 def test_batch_uneven_jobs_vs_oracle(dto, O, gpu):
@@ -68,11 +67,9 @@ def test_batch_uneven_jobs_vs_oracle(dto, O, gpu):
     b = dto.Batch(jobs, n_perm, seed=seed)
     b.autotune()
     b.launch()
-    first = [(r.unpermuted.rank1, r.unpermuted.rank2, r.unpermuted.intersection_size, r.unpermuted.pvalue, r.n_le)
-             for r in b.results()]
+    first = [(rec_tuple(r.unpermuted), r.n_le) for r in b.results()]
     b.launch()
-    again = [(r.unpermuted.rank1, r.unpermuted.rank2, r.unpermuted.intersection_size, r.unpermuted.pvalue, r.n_le)
-             for r in b.results()]
+    again = [(rec_tuple(r.unpermuted), r.n_le) for r in b.results()]
     assert again == first
     for j, (ids1, r1, ids2, r2, _) in enumerate(specs):
         l1, l2 = O.OracleList(ids1, r1), O.OracleList(ids2, r2)
@@ -83,7 +80,7 @@ def test_batch_uneven_jobs_vs_oracle(dto, O, gpu):
         ub = O.optimize(l1, l2, N, fast=True)
         ob = O.optimize_many(l1, l2, N, i1, i2, fast=True)
         le = sum(1 for x in ob if x.pvalue <= ub.pvalue)
-        assert first[j] == (ub.rank1, ub.rank2, ub.intersection_size, ub.pvalue, le), (j, len(ids1))
+        assert first[j] == (rec_tuple(ub), le), (j, len(ids1))
 
 
 def test_server_mix_with_large_jobs(dto, O, gpu):
@@ -117,6 +114,85 @@ def test_server_mix_with_large_jobs(dto, O, gpu):
         ob = O.optimize_many(l1, l2, N, i1, i2, fast=True)
         le = sum(1 for x in ob if x.pvalue <= ub.pvalue)
         r = got[rid]
-        assert (r.unpermuted.rank1, r.unpermuted.rank2, r.unpermuted.intersection_size, r.unpermuted.pvalue,
-                r.n_le) == (ub.rank1, ub.rank2, ub.intersection_size, ub.pvalue, le), (rid, N)
+        assert (rec_tuple(r.unpermuted), r.n_le) == (rec_tuple(ub), le), (rid, N)
     sv.close()
+
+
+def test_concurrent_launches_of_one_large_job(dto, O, gpu):
+    """A job whose pair staging lives in global memory (N = 36,000), launched
+    on two streams at once with different permutations: the launches share
+    the job's staging buffer, so the library orders them; both outputs equal
+    the oracle's."""
+    import torch
+    from dual_threshold_optimization_amd import _lib
+    from dual_threshold_optimization_amd import workloads as W
+    n = 36000
+    ids1, r1, ids2, r2, bg = _genome_like(W, n, 77)
+    job = dto.Job.from_lists(ids1, r1, ids2, r2, background=bg, drop_outside_background=True)
+    l1, l2 = _filtered_oracle_lists(O, ids1, r1, ids2, r2, bg)
+    dev = torch.device("cuda:0")
+    P = 3
+    runs = []
+    for seed in (5, 6):
+        i1, i2 = W.explicit_permutations(P, n, n, seed)
+        runs.append((i1, i2, torch.from_numpy(i1.astype(np.int32)).to(dev), torch.from_numpy(i2.astype(np.int32)).to(dev),
+                     torch.zeros(P * _lib.RECORD_DTYPE.itemsize, dtype=torch.uint8, device=dev), torch.cuda.Stream()))
+    torch.cuda.synchronize()
+    for _ in range(2):
+        for i1, i2, d1, d2, out, st in runs:
+            job.run_device(P, out.data_ptr(), d_idx1=d1.data_ptr(), d_idx2=d2.data_ptr(), stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    for i1, i2, d1, d2, out, st in runs:
+        got = np.frombuffer(out.cpu().numpy().tobytes(), dtype=_lib.RECORD_DTYPE)
+        want = O.optimize_many(l1, l2, n, i1, i2, fast=True)
+        for p in range(P):
+            assert rec_tuple(got[p]) == rec_tuple(want[p]), p
+
+
+def test_server_pool_admission_deferred_jobs(dto, O, gpu):
+    """Jobs built with defer_upload stay on the host until admitted; a small
+    HBM budget admits a few at a time (the rest wait in the queue), and
+    requests of different launch shapes run in their own slices.  Every
+    result equals the oracle's."""
+    from dual_threshold_optimization_amd import workloads as W
+    sizes = [900, 2500, 700, 22000, 1500, 600, 4000, 1100]
+    # 1 MiB: the 22,000-feature job's tables (~0.6 MB) fit only beside a few small ones
+    sv = dto.Server(hbm_budget_bytes=1 << 20, perms_per_slice=3, seed=9)
+    want = {}
+    for j, n in enumerate(sizes):
+        ids1, r1, ids2, r2, bg = W.serving_job_lists(n, int(0.03 * n), 0.9, 900 + j)
+        job = dto.Job.from_lists(ids1, r1, ids2, r2, background=bg, drop_outside_background=True, defer_upload=True)
+        P = 2 + (j % 4)
+        rid = sv.submit(job, P)
+        want[rid] = (ids1, r1, ids2, r2, bg, P)
+    q, a = sv.pending()
+    assert q == len(sizes) and a == 0
+    got = {}
+    for _ in range(4000):
+        for rid, res, lat in sv.poll(timeout_ms=50):
+            got[rid] = res
+        if len(got) == len(want):
+            break
+    assert len(got) == len(want)
+    assert sv.pending() == (0, 0)
+    for rid, (ids1, r1, ids2, r2, bg, P) in want.items():
+        l1, l2 = _filtered_oracle_lists(O, ids1, r1, ids2, r2, bg)
+        N = len(bg)
+        i1 = np.stack([dto.philox_permutation(9, rid, p, 1, l1.n) for p in range(P)])
+        i2 = np.stack([dto.philox_permutation(9, rid, p, 2, l2.n) for p in range(P)])
+        ub = O.optimize(l1, l2, N, fast=True)
+        ob = O.optimize_many(l1, l2, N, i1, i2, fast=True)
+        le = sum(1 for x in ob if x.pvalue <= ub.pvalue)
+        r = got[rid]
+        assert (rec_tuple(r.unpermuted), r.n_le) == (rec_tuple(ub), le), (rid, N)
+    sv.close()
+
+
+def test_deferred_job_runs_on_first_use(dto, O, gpu):
+    """A deferred job used outside a server uploads itself on first use."""
+    from dual_threshold_optimization_amd import workloads as W
+    ids1, r1, ids2, r2, _ = W.planted_lists(800, 30, 100, 80, 0.05, seed=5)
+    job = dto.Job.from_lists(ids1, r1, ids2, r2, defer_upload=True)
+    l1, l2 = O.OracleList(ids1, r1), O.OracleList(ids2, r2)
+    N = O.population_size(l1, l2)
+    assert rec_tuple(job.optimize()) == rec_tuple(O.optimize(l1, l2, N, fast=True))

@@ -148,6 +148,7 @@ def test_readme_kat(O):
     assert N == 30
     best = O.optimize(l1, l2, N)
     assert best.as_tuple() == (24, 13, 24, 13, 30, 12, 0.15632183908046102)
+    assert best.tie_counts() == (1, 1)  # README.md:174-186 shows no tie warning
     assert O.optimize(l1, l2, N, fast=True).as_tuple() == best.as_tuple()
     # the exact-rational optimum is the same cell
     ex = min(((exact_sf(30, A, B, k), -k, a, b) for a, b, A, B, k in brute_grid(i1, r1, i2, r2, 30)))
@@ -189,6 +190,34 @@ def test_grid_vs_brute_force(O, seed):
     best = O.optimize(l1, l2, N, idx1, idx2)
     key = min((r.pvalue, -r.intersection_size, r.rank1, r.rank2) for r in recs)
     assert (best.pvalue, -best.intersection_size, best.rank1, best.rank2) == key
+    # tie counts behind optimize_main.rs:86-108's two warnings
+    at_min = [r for r in recs if r.pvalue == key[0]]
+    n_pk = sum(1 for r in at_min if r.intersection_size == -key[1])
+    assert best.tie_counts() == (len(at_min), n_pk)
+    assert O.optimize(l1, l2, N, idx1, idx2, fast=True).tie_counts() == (len(at_min), n_pk)
+
+
+def test_tie_counts_p_one_and_p_zero_classes(O):
+    """Grids whose minimum is shared: every cell at p == 1 (disjoint sets),
+    and p == 0 cells (empty tails from list-2 duplicates); the counts follow
+    the cells' (p, k) exactly."""
+    # no shared feature anywhere within the thresholds: every k = 0, every p = 1
+    l1 = O.OracleList(["a", "b", "c", "d"], [1, 2, 3, 4])
+    l2 = O.OracleList(["x", "y", "z", "w"], [1, 2, 3, 4])
+    recs = O.process_threshold_pairs(l1, l2, 12)
+    b = O.optimize(l1, l2, 12)
+    assert b.pvalue == 1.0 and all(r.pvalue == 1.0 for r in recs)
+    kmax = max(r.intersection_size for r in recs)
+    assert b.tie_counts() == (len(recs), sum(1 for r in recs if r.intersection_size == kmax))
+    # list-2 duplicates of one list-1 id: k > K cells have p == 0 exactly
+    l1 = O.OracleList(["a", "b", "c"], [1, 2, 3])
+    l2 = O.OracleList(["a", "a", "a", "b"], [1, 2, 3, 4])
+    recs = O.process_threshold_pairs(l1, l2, 5)
+    b = O.optimize(l1, l2, 5)
+    zero = [r for r in recs if r.pvalue == 0.0]
+    assert b.pvalue == 0.0 and len(zero) >= 2
+    kmax = max(r.intersection_size for r in zero)
+    assert b.tie_counts() == (len(zero), sum(1 for r in zero if r.intersection_size == kmax))
 
 
 @pytest.mark.parametrize("seed", range(8))
