@@ -28,6 +28,7 @@ DTO_ERR_BUSY = 11
 
 DTO_DROP_OUTSIDE_BACKGROUND = 0x1
 DTO_DEFER_UPLOAD = 0x2
+DTO_NOALLOC_MAX_CELLS = 1 << 22
 DTO_MODE_SCREENED = 0
 DTO_MODE_PLAIN = 1
 MODES = {"screened": DTO_MODE_SCREENED, "plain": DTO_MODE_PLAIN}
@@ -119,6 +120,7 @@ SIGNATURES = {
     "dto_job_destroy": (None, [_vp]),
     "dto_optimize": (_int, [_vp, _P(_u32), _P(_u32), _u32, _P(Record)]),
     "dto_process_threshold_pairs": (_int, [_vp, _P(_u32), _P(_u32), _vp]),
+    "dto_process_threshold_pairs_noalloc": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "dto_run": (_int, [_vp, _u64, _u64, _P(_u32), _P(_u32), _u32, _P(Result), _vp]),
     "dto_run_device": (_int, [_vp, _u64, _u64, _u64, _vp, _vp, _u32, _vp, _vp]),
     "dto_philox_permutation": (_int, [_u64, _u32, _u32, _u32, _u32, _P(_u32)]),

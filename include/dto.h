@@ -162,6 +162,20 @@ int dto_optimize(dto_job *job, const uint32_t *idx1, const uint32_t *idx2, uint3
  * permutation: all T1*T2 records, row-major over (threshold1, threshold2). */
 int dto_process_threshold_pairs(dto_job *job, const uint32_t *idx1, const uint32_t *idx2,
                                 dto_record *out);
+/* No counterpart in the reference: dto_process_threshold_pairs for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It allocates nothing:
+ * d_idx1/d_idx2 (device arrays of len1/len2 entries, or NULL for unpermuted)
+ * and d_out (a device array of T1*T2 records) are the caller's, and the
+ * overlap counts go to a fixed device array of DTO_NOALLOC_MAX_CELLS words
+ * that each GPU holds for the life of the process.  Grids larger than that
+ * fail with DTO_ERR_TOO_LARGE.  Asynchronous on stream (a hipStream_t, NULL =
+ * default); d_status (device int, may be NULL) is cleared and then set
+ * nonzero if a permutation index is out of range.  Every call on one GPU
+ * shares that array: calls that overlap in time on one GPU race. */
+#define DTO_NOALLOC_MAX_CELLS (1u << 22) /* 16 MiB: a 2048 x 2048 threshold grid */
+int dto_process_threshold_pairs_noalloc(dto_job *job, const uint32_t *d_idx1, const uint32_t *d_idx2,
+                                        dto_record *d_out, int *d_status, void *stream);
 /* The CLI's run (src/main.rs:122-126, 156-165): task 0 unpermuted plus
  * n_perm permutations, run_single_node (src/run/single_node.rs:83-137) on the
  * GPU, then empirical_pvalue.  idx1/idx2: host arrays of n_perm*len1 and
