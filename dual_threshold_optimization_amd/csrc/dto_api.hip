@@ -59,7 +59,7 @@ int dto::device_props(int dev, DevProps *out) {
 /* ------------------------------------------------------------------ */
 static uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 
-LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limit) {
+LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limit, bool nb_lds) {
   LdsLayout L{};
   const uint32_t N1 = (uint32_t)h.l1.ids.size(), N2 = (uint32_t)h.l2.ids.size();
   const uint32_t T1 = (uint32_t)h.thr1.size(), T2 = (uint32_t)h.thr2.size();
@@ -73,7 +73,8 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
     field = fixed;
     fixed = align16(fixed + bytes);
   };
-  region(L.nb, 2u * std::max(std::max(N1, N2), 1u));
+  region(L.nb, nb_lds ? 2u * std::max(std::max(N1, N2), 1u) : 0u);
+  L.nb_in_lds = nb_lds ? 1u : 0u;
   region(L.rowstart, 4u * (T1 + 1));
   /* cursor is used while building the rows, cand0 in the reverse sweep */
   region(L.cursor, std::max(4u * T1, 8u * (uint32_t)W * (uint32_t)kCand0Cap));
@@ -120,6 +121,14 @@ static size_t put(std::vector<unsigned char> &buf, const std::vector<T> &v) {
   return off;
 }
 
+LdsLayout dto::screen_layout(const HostJob &h, int W, size_t lim) {
+  LdsLayout L = make_layout(h, W, true, lim, true);
+  if (L.total <= lim) return L;
+  L = make_layout(h, W, true, lim, false);
+  if (L.total <= lim) return L;
+  return make_layout(h, W, false, lim, true);
+}
+
 int prepare_job(dto_job *j) {
   int dev;
   int rc = current_device(&dev);
@@ -129,12 +138,12 @@ int prepare_job(dto_job *j) {
   const HostJob &h = j->h;
   /* LDS: keep the screen table resident when it fits */
   const size_t lim = pr.lds_per_block;
-  LdsLayout with_h = make_layout(h, kNT / 64, true, lim), without_h = make_layout(h, kNT / 64, false, lim);
+  LdsLayout without_h = make_layout(h, kNT / 64, false, lim);
   if (without_h.total > lim)
     return fail(DTO_ERR_TOO_LARGE, "job needs " + std::to_string(without_h.total) + " B of LDS per workgroup, device has " +
                                        std::to_string(lim));
   j->lo_plain = without_h;
-  j->lo_screen = with_h.total <= lim ? with_h : without_h;
+  j->lo_screen = screen_layout(h, kNT / 64, lim);
   std::vector<unsigned char> &buf = j->image;
   buf.clear();
   ArenaOffsets &o = j->off;
@@ -214,9 +223,17 @@ int upload_job(dto_job *j) {
     (void)hipFree(j->ws);
     j->ws = nullptr;
   }
+  if (j->gnb) {
+    (void)hipFree(j->gnb);
+    j->gnb = nullptr;
+  }
+  j->ws_slots = (uint32_t)pr.cus * 4u;
   if (!j->lo_plain.pairs_in_lds || !j->lo_screen.pairs_in_lds) {
-    j->ws_slots = (uint32_t)pr.cus * 4u;
     HIP_TRY(hipMalloc(&j->ws, (size_t)j->ws_slots * std::max<uint32_t>(j->dev.N2, 1) * sizeof(uint16_t)));
+    if (!j->ws_ev) HIP_TRY(hipEventCreateWithFlags(&j->ws_ev, hipEventDisableTiming));
+  }
+  if (!j->lo_screen.nb_in_lds) {
+    HIP_TRY(hipMalloc(&j->gnb, (size_t)j->ws_slots * std::max(std::max(j->dev.N1, j->dev.N2), 1u) * sizeof(uint16_t)));
     if (!j->ws_ev) HIP_TRY(hipEventCreateWithFlags(&j->ws_ev, hipEventDisableTiming));
   }
   return DTO_OK;
@@ -230,9 +247,9 @@ int ensure_resident(dto_job *j) {
 /* ------------------------------------------------------------------ */
 /* launches                                                              */
 /* ------------------------------------------------------------------ */
-/* Launches that stage their pairs in the job's global buffer (lists too long
- * for the LDS) run one after another, whatever their streams: each waits for
- * the previous one's completion event.  Inside a stream capture the wait is
+/* Launches that stage their pairs or keep their bins in the job's global
+ * buffers (lists too long for the LDS) run one after another, whatever their
+ * streams: each waits for the previous one's completion event.  Inside a stream capture the wait is
  * left out (a capture cannot wait on work recorded outside it), so captured
  * launches of one such job must not overlap. */
 struct StagingOrder {
@@ -240,7 +257,7 @@ struct StagingOrder {
   hipStream_t st;
   bool on = false;
   StagingOrder(const dto_job *job, const LdsLayout &lo, hipStream_t s) : j(const_cast<dto_job *>(job)), st(s) {
-    if (lo.pairs_in_lds || !j->ws_ev) return;
+    if ((lo.pairs_in_lds && lo.nb_in_lds) || !j->ws_ev) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cs) != hipSuccess) (void)hipGetLastError();
     if (cs != hipStreamCaptureStatusNone) return;
@@ -255,18 +272,18 @@ struct StagingOrder {
   }
 };
 
-template <bool E, bool HL, bool PL, bool PA, bool RC>
+template <bool E, bool HL, bool PL, bool PA, bool RC, bool NBL = true>
 static hipError_t launch_opt(const dto_job *j, const LdsLayout &lo, uint32_t grid, hipStream_t st,
                              const uint32_t *i1, const uint32_t *i2, uint64_t n_perm, uint64_t base, uint64_t seed,
                              uint32_t job_id, int32_t permuted, dto_record *out) {
-  auto kern = k_optimize<kNT, E, HL, PL, PA, RC>;
+  auto kern = k_optimize<kNT, E, HL, PL, PA, RC, NBL>;
   (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lo.total);
   /* a slot of the job's counter ring for this launch, zeroed on its stream */
   uint32_t *ctr = j->d_ctr + (const_cast<dto_job *>(j)->ctr_next.fetch_add(1u) % kCounterRing);
   StagingOrder order(j, lo, st);
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, ctr);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kNT), lo.total, st, j->dev, lo, i1, i2, n_perm, base, seed, job_id,
-                     permuted, out, j->d_err, j->ws, ctr);
+                     permuted, out, j->d_err, j->ws, j->gnb, ctr);
   return hipGetLastError();
 }
 
@@ -282,6 +299,11 @@ static hipError_t launch_opt_r(const dto_job *j, const LdsLayout &lo, uint32_t g
                ? launch_opt<E, false, true, true, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out)
                : launch_opt<E, false, true, false, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
   const bool rc = regcols(j);
+  if (HL && !lo.nb_in_lds) /* the table in LDS, the bins global (pairs alias the table) */
+    return rc ? launch_opt<E, true, false, true, true, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id,
+                                                              permuted, out)
+              : launch_opt<E, true, false, true, false, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id,
+                                                               permuted, out);
   if (lo.pairs_in_lds) {
     if (rc) return launch_opt<E, HL, false, true, true>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
     return launch_opt<E, HL, false, true, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
@@ -298,7 +320,7 @@ static uint32_t perm_grid(const dto_job *j, const LdsLayout &lo, uint64_t n_perm
   uint64_t per_cu = std::max<uint64_t>(1, pr.lds_per_block ? (160u * 1024u) / std::max<uint32_t>(lo.total, 1) : 1);
   per_cu = std::min<uint64_t>(per_cu, 4);
   uint64_t g = std::min<uint64_t>(n_perm, (uint64_t)pr.cus * per_cu);
-  if (!lo.pairs_in_lds) g = std::min<uint64_t>(g, j->ws_slots); /* one staging slot per workgroup */
+  if (!lo.pairs_in_lds || !lo.nb_in_lds) g = std::min<uint64_t>(g, j->ws_slots); /* one global slot per workgroup */
   return (uint32_t)std::max<uint64_t>(g, 1);
 }
 
@@ -638,6 +660,7 @@ void dto_job_destroy(dto_job *job) {
   if (!job) return;
   if (job->arena) (void)hipFree(job->arena);
   if (job->ws) (void)hipFree(job->ws);
+  if (job->gnb) (void)hipFree(job->gnb);
   if (job->ws_ev) (void)hipEventDestroy(job->ws_ev);
   delete job;
 }

@@ -39,75 +39,92 @@ namespace {
 
 /* Launch shapes the batch kernels are built for: threads per workgroup, the
  * screen table in LDS or not, the pair staging in LDS or global memory. */
-template <int NT, bool HL, bool PL, bool RC>
+/* Global buffers of a launch: pair staging (ws, ws_stride u16 per
+ * workgroup) and bins arrays (gnb, gnb_stride u16 per workgroup). */
+struct Globals {
+  uint16_t *ws = nullptr, *gnb = nullptr;
+  uint32_t ws_stride = 0, gnb_stride = 0;
+};
+
+template <int NT, bool HL, bool PL, bool RC, bool NBL>
 hipError_t launch_batch_t(uint32_t grid, uint32_t lds, hipStream_t st, const BatchJob *jobs, const WorkItem *items,
-                          uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, uint16_t *ws,
-                          uint32_t ws_stride) {
-  auto kern = k_batch<NT, HL, PL, RC>;
+                          uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, const Globals &g) {
+  auto kern = k_batch<NT, HL, PL, RC, NBL>;
   (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   /* every layout ends with its 16-byte misc slot: the largest one's is free in all */
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride,
-                     lds - 16u);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, jobs, items, n_items, seed, counter, err, g.ws, g.ws_stride,
+                     g.gnb, g.gnb_stride, lds - 16u);
   return hipGetLastError();
 }
 
 template <int NT>
-hipError_t launch_batch_n(bool hl, bool pl, bool rc, uint32_t grid, uint32_t lds, hipStream_t st,
+hipError_t launch_batch_n(bool hl, bool nbl, bool pl, bool rc, uint32_t grid, uint32_t lds, hipStream_t st,
                           const BatchJob *jobs, const WorkItem *items, uint32_t n_items, uint64_t seed,
-                          uint32_t *counter, int *err, uint16_t *ws, uint32_t ws_stride) {
+                          uint32_t *counter, int *err, const Globals &g) {
+  if (hl && nbl)
+    return rc ? launch_batch_t<NT, true, true, true, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, g)
+              : launch_batch_t<NT, true, true, false, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, g);
   if (hl)
-    return rc ? launch_batch_t<NT, true, true, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride)
-              : launch_batch_t<NT, true, true, false>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride);
+    return rc ? launch_batch_t<NT, true, true, true, false>(grid, lds, st, jobs, items, n_items, seed, counter, err, g)
+              : launch_batch_t<NT, true, true, false, false>(grid, lds, st, jobs, items, n_items, seed, counter, err, g);
   if (pl)
-    return rc ? launch_batch_t<NT, false, true, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride)
-              : launch_batch_t<NT, false, true, false>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride);
-  return rc ? launch_batch_t<NT, false, false, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride)
-            : launch_batch_t<NT, false, false, false>(grid, lds, st, jobs, items, n_items, seed, counter, err, ws, ws_stride);
+    return rc ? launch_batch_t<NT, false, true, true, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, g)
+              : launch_batch_t<NT, false, true, false, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, g);
+  return rc ? launch_batch_t<NT, false, false, true, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, g)
+            : launch_batch_t<NT, false, false, false, true>(grid, lds, st, jobs, items, n_items, seed, counter, err, g);
 }
 
 /* Batch-wide launch parameters for a set of jobs at nt threads. */
 struct Shape {
   int nt = 512;
-  bool hl = true;
-  bool pl = true;
-  bool rc = true; /* every job's lane segment fits in registers */
+  bool hl = true;  /* screen table in LDS */
+  bool nbl = true; /* bins arrays in LDS */
+  bool pl = true;  /* pair staging in LDS */
+  bool rc = true;  /* every job's lane segment fits in registers */
   uint32_t lds = 0;
-  uint32_t max_n2 = 0;
+  uint32_t max_n2 = 0, max_n = 0; /* largest N2, largest max(N1, N2) */
   std::vector<LdsLayout> lo;
 };
 
 hipError_t launch_batch(const Shape &sh, uint32_t grid, hipStream_t st, const BatchJob *jobs, const WorkItem *items,
-                        uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, uint16_t *ws) {
+                        uint32_t n_items, uint64_t seed, uint32_t *counter, int *err, const Globals &g) {
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, counter, err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (sh.nt == 256)
-    return launch_batch_n<256>(sh.hl, sh.pl, sh.rc, grid, sh.lds, st, jobs, items, n_items, seed, counter, err, ws,
-                               sh.max_n2);
-  return launch_batch_n<512>(sh.hl, sh.pl, sh.rc, grid, sh.lds, st, jobs, items, n_items, seed, counter, err, ws,
-                             sh.max_n2);
+    return launch_batch_n<256>(sh.hl, sh.nbl, sh.pl, sh.rc, grid, sh.lds, st, jobs, items, n_items, seed, counter,
+                               err, g);
+  return launch_batch_n<512>(sh.hl, sh.nbl, sh.pl, sh.rc, grid, sh.lds, st, jobs, items, n_items, seed, counter, err,
+                             g);
 }
 
+/* One shape for all jobs of a launch: the first of (table and bins in
+ * LDS), (table in LDS, bins global), (table global) that every job fits. */
 int plan_shape(const std::vector<dto_job *> &jobs, int nt, size_t lds_limit, Shape &s) {
   s.nt = nt;
-  bool fit_h = true, fit_p = true;
+  const int W = nt / 64;
+  bool fit_hb = true, fit_h = true, fit_p = true;
   s.max_n2 = 1;
+  s.max_n = 1;
   s.rc = true;
   for (dto_job *j : jobs) {
     if ((j->dev.T2w + 63) / 64 > kSegWords) s.rc = false;
-    if (make_layout(j->h, nt / 64, true, lds_limit).total > lds_limit) fit_h = false;
+    if (make_layout(j->h, W, true, lds_limit, true).total > lds_limit) fit_hb = false;
+    if (make_layout(j->h, W, true, lds_limit, false).total > lds_limit) fit_h = false;
     s.max_n2 = std::max(s.max_n2, j->dev.N2);
+    s.max_n = std::max(s.max_n, std::max(j->dev.N1, j->dev.N2));
   }
+  s.hl = fit_h;
+  s.nbl = fit_hb || !fit_h;
   s.lo.clear();
   s.lds = 0;
   for (dto_job *j : jobs) {
-    LdsLayout L = make_layout(j->h, nt / 64, fit_h, lds_limit);
+    LdsLayout L = make_layout(j->h, W, s.hl, lds_limit, s.nbl);
     if (L.total > lds_limit) return fail(DTO_ERR_TOO_LARGE, "job does not fit in LDS");
     if (!L.pairs_in_lds) fit_p = false;
     s.lo.push_back(L);
     s.lds = std::max(s.lds, L.total);
   }
-  s.hl = fit_h;
   s.pl = fit_p;
   return DTO_OK;
 }
@@ -147,7 +164,8 @@ struct dto_batch {
   uint32_t *d_counter = nullptr;
   int *d_err = nullptr;
   dto_record *d_out = nullptr;
-  uint16_t *d_ws = nullptr; /* global pair staging when the LDS cannot hold the pairs */
+  uint16_t *d_ws = nullptr;  /* global pair staging when the LDS cannot hold the pairs */
+  uint16_t *d_gnb = nullptr; /* global bins arrays when the LDS holds the table but not the bins */
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t own_stream = nullptr;
@@ -161,9 +179,20 @@ static void batch_free_plan(dto_batch *b) {
   if (b->d_jobs) (void)hipFree(b->d_jobs);
   if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_ws) (void)hipFree(b->d_ws);
+  if (b->d_gnb) (void)hipFree(b->d_gnb);
   b->d_jobs = nullptr;
   b->d_items = nullptr;
   b->d_ws = nullptr;
+  b->d_gnb = nullptr;
+}
+
+static Globals batch_globals(const dto_batch *b) {
+  Globals g;
+  g.ws = b->d_ws;
+  g.ws_stride = b->shape.max_n2;
+  g.gnb = b->d_gnb;
+  g.gnb_stride = b->shape.max_n;
+  return g;
 }
 
 /* Builds the job table and the work queue for the current shape and chunk. */
@@ -205,6 +234,7 @@ static int batch_plan(dto_batch *b) {
   if (!items.empty()) HIP_TRY(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
   b->grid = grid_for(b->cus, b->shape.lds, b->shape.nt, b->n_items);
   if (!b->shape.pl) HIP_TRY(hipMalloc(&b->d_ws, (size_t)b->grid * b->shape.max_n2 * sizeof(uint16_t)));
+  if (!b->shape.nbl) HIP_TRY(hipMalloc(&b->d_gnb, (size_t)b->grid * b->shape.max_n * sizeof(uint16_t)));
   return DTO_OK;
 }
 
@@ -264,7 +294,7 @@ int dto_batch_launch(dto_batch *b, void *stream) {
     /* capture queue reset + persistent kernel once, replay afterwards */
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipError_t e = launch_batch(b->shape, b->grid, st, b->d_jobs, b->d_items, b->n_items, b->seed, b->d_counter,
-                                b->d_err, b->d_ws);
+                                b->d_err, batch_globals(b));
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(st, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -317,7 +347,7 @@ int dto_batch_autotune(dto_batch *b, int *waves) {
       for (int rep = 0; rep < 2 && good; ++rep) {
         HIP_TRY(hipEventRecord(e0, 0));
         if (launch_batch(b->shape, b->grid, 0, b->d_jobs, b->d_items, b->n_items, b->seed, b->d_counter, b->d_err,
-                         b->d_ws) != hipSuccess)
+                         batch_globals(b)) != hipSuccess)
           good = false;
         HIP_TRY(hipEventRecord(e1, 0));
         HIP_TRY(hipEventSynchronize(e1));
@@ -412,15 +442,18 @@ struct Pool {
  * Requests of one class share slices; a large request never moves small
  * ones onto its slower shape or its lower occupancy. */
 struct ShapeClass {
-  bool hl, pl, rc;
+  bool hl, nbl, pl, rc;
   uint32_t per_cu;
-  bool operator==(const ShapeClass &o) const { return hl == o.hl && pl == o.pl && rc == o.rc && per_cu == o.per_cu; }
+  bool operator==(const ShapeClass &o) const {
+    return hl == o.hl && nbl == o.nbl && pl == o.pl && rc == o.rc && per_cu == o.per_cu;
+  }
 };
 
 ShapeClass class_of(const dto_job *j) {
   const LdsLayout &L = j->lo_screen;
   ShapeClass c;
   c.hl = L.h_in_lds != 0;
+  c.nbl = L.nb_in_lds != 0;
   c.pl = L.pairs_in_lds != 0;
   c.rc = (j->dev.T2w + 63) / 64 <= kSegWords;
   c.per_cu = std::min<uint32_t>(4, std::max<uint32_t>(1, (160u * 1024u) / std::max<uint32_t>(L.total, 1)));
@@ -461,6 +494,7 @@ struct Slot {
   uint32_t *d_counter = nullptr;
   int *d_err = nullptr;
   size_t ws_off = kNoRegion, ws_bytes = 0;
+  size_t gnb_off = kNoRegion, gnb_bytes = 0;
   std::vector<Part> parts;
   bool inflight = false;
 };
@@ -490,7 +524,7 @@ static void server_free_request(dto_server *sv, Request *r) {
   if (r->out_off != kNoRegion) sv->pool.release(r->out_off, r->out_bytes);
   if (r->job_off != kNoRegion) sv->pool.release(r->job_off, r->job_bytes);
   if (r->job) {
-    if (r->job->arena || r->job->ws)
+    if (r->job->arena || r->job->ws || r->job->gnb)
       sv->graveyard.push_back(r->job); /* its hipFree would wait for the device */
     else
       dto_job_destroy(r->job);
@@ -558,7 +592,7 @@ static int server_launch_slice(dto_server *sv, Slot &s) {
     if (!r->failed && r->next_task < r->n_perm + 1 && (!oldest || r->t_submit < oldest->t_submit)) oldest = r;
   if (!oldest) return DTO_OK;
   const ShapeClass cls = oldest->cls;
-  uint32_t n_items = 0, lds = 0, max_n2 = 1;
+  uint32_t n_items = 0, lds = 0, max_n2 = 1, max_n = 1;
   for (Request *r : sv->active) {
     if (r->failed || !(r->cls == cls) || r->next_task >= r->n_perm + 1) continue;
     const uint32_t c = (uint32_t)std::min<uint64_t>(sv->cfg.perms_per_slice, r->n_perm + 1 - r->next_task);
@@ -568,6 +602,7 @@ static int server_launch_slice(dto_server *sv, Slot &s) {
     n_items += its;
     lds = std::max(lds, r->job->lo_screen.total);
     max_n2 = std::max(max_n2, r->job->dev.N2);
+    max_n = std::max(max_n, std::max(r->job->dev.N1, r->job->dev.N2));
   }
   if (s.parts.empty()) return DTO_OK;
   const uint32_t grid = grid_for(sv->cus, lds, kNT, n_items);
@@ -575,6 +610,16 @@ static int server_launch_slice(dto_server *sv, Slot &s) {
     s.ws_bytes = (size_t)grid * max_n2 * sizeof(uint16_t);
     if (!sv->pool.alloc(s.ws_bytes, &s.ws_off)) {
       s.parts.clear(); /* no room for the staging now: try again later */
+      s.ws_off = kNoRegion;
+      return DTO_OK;
+    }
+  }
+  if (!cls.nbl) {
+    s.gnb_bytes = (size_t)grid * max_n * sizeof(uint16_t);
+    if (!sv->pool.alloc(s.gnb_bytes, &s.gnb_off)) {
+      s.parts.clear();
+      s.gnb_off = kNoRegion;
+      if (s.ws_off != kNoRegion) sv->pool.release(s.ws_off, s.ws_bytes);
       s.ws_off = kNoRegion;
       return DTO_OK;
     }
@@ -606,11 +651,17 @@ static int server_launch_slice(dto_server *sv, Slot &s) {
   shape.nt = kNT;
   shape.hl = cls.hl;
   shape.pl = cls.pl;
+  shape.nbl = cls.nbl;
   shape.rc = cls.rc;
   shape.lds = lds;
   shape.max_n2 = max_n2;
-  uint16_t *ws = cls.pl ? nullptr : (uint16_t *)(sv->pool.base + s.ws_off);
-  hipError_t e = launch_batch(shape, grid, s.st, s.d_jobs, s.d_items, n_items, sv->cfg.seed, s.d_counter, s.d_err, ws);
+  shape.max_n = max_n;
+  Globals g;
+  g.ws = cls.pl ? nullptr : (uint16_t *)(sv->pool.base + s.ws_off);
+  g.ws_stride = max_n2;
+  g.gnb = cls.nbl ? nullptr : (uint16_t *)(sv->pool.base + s.gnb_off);
+  g.gnb_stride = max_n;
+  hipError_t e = launch_batch(shape, grid, s.st, s.d_jobs, s.d_items, n_items, sv->cfg.seed, s.d_counter, s.d_err, g);
   if (e != hipSuccess) return fail(DTO_ERR_DEVICE, std::string("slice launch: ") + hipGetErrorString(e));
   HIP_TRY(hipMemcpyAsync(s.h_err, s.d_err, sizeof(int), hipMemcpyDeviceToHost, s.st));
   HIP_TRY(hipEventRecord(s.ev, s.st));
@@ -712,7 +763,8 @@ int dto_server_poll(dto_server *sv, uint32_t timeout_ms, uint64_t *ids, dto_resu
       }
       s.parts.clear();
       if (s.ws_off != kNoRegion) sv->pool.release(s.ws_off, s.ws_bytes);
-      s.ws_off = kNoRegion;
+      if (s.gnb_off != kNoRegion) sv->pool.release(s.gnb_off, s.gnb_bytes);
+      s.ws_off = s.gnb_off = kNoRegion;
       s.inflight = false; /* the next launch on this slot clears its error word */
     }
     /* complete (or drop failed) requests */

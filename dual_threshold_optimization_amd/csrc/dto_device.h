@@ -44,7 +44,7 @@ struct LdsLayout {
   uint32_t h;        /* f32[N+1] screen table (only when h_in_lds); always 0 */
   uint32_t pairs;    /* u16[N2] row of each list2 position's pair (staging): aliases h; always 0 */
   uint32_t pairs_in_lds;
-  uint32_t nb;       /* u16[max(N1,N2)]: list1 bins after the shuffle, then row-sorted list2 bins */
+  uint32_t nb;       /* u16[max(N1,N2)]: list1 bins after the shuffle, then row-sorted list2 bins (nb_in_lds) */
   uint32_t rowstart; /* u32[T1+1] */
   uint32_t cursor;   /* u32[T1] */
   uint32_t cw;       /* u32[W][T2w_pad] per-wave running column counts (2 x u16 per word) */
@@ -60,6 +60,7 @@ struct LdsLayout {
   uint32_t total;
   uint32_t T2w_pad;
   uint32_t h_in_lds;
+  uint32_t nb_in_lds; /* 0: the bins array is a per-workgroup global slice (leaves the LDS to h) */
 };
 
 constexpr int kCand0Cap = 128;

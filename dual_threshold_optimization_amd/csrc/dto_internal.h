@@ -28,7 +28,12 @@ struct DevProps {
 };
 int current_device(int *dev);
 int device_props(int dev, DevProps *out);
-LdsLayout make_layout(const HostJob &h, int W, bool with_h, size_t lds_limit);
+/* with_h: the screen table in LDS; nb_lds: the bins array in LDS (else a
+ * per-workgroup global slice of max(N1, N2) u16). */
+LdsLayout make_layout(const HostJob &h, int W, bool with_h, size_t lds_limit, bool nb_lds = true);
+/* The screened kernels' layout for a job: table and bins in LDS when they
+ * fit, else the table in LDS and the bins global, else the table global. */
+LdsLayout screen_layout(const HostJob &h, int W, size_t lds_limit);
 }  // namespace dto
 
 constexpr uint32_t kCounterRing = 256;
@@ -59,6 +64,7 @@ struct dto_job {
   int device = -1;
   dto::LdsLayout lo_screen{}, lo_plain{};
   uint16_t *ws = nullptr; /* global pair staging (only when the LDS cannot hold the pairs) */
+  uint16_t *gnb = nullptr; /* global bins arrays, one per workgroup (only when lo_screen.nb_in_lds == 0) */
   uint32_t ws_slots = 0;
   /* launches using ws run in order: each waits for the previous one's event */
   hipEvent_t ws_ev = nullptr;

@@ -199,13 +199,16 @@ struct Lds {
   int *ushare; /* the reverse sweep's best upper bound of the minimum log p, as an ordered key */
 };
 
-__device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo, int W) {
+/* NBL false: the bins array is the workgroup's slice of a global buffer
+ * (gnb), which leaves the LDS to the screen table on larger jobs. */
+template <bool NBL = true>
+__device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo, int W, uint16_t *gnb = nullptr) {
   Lds s;
   /* the screen table and the pair staging start the dynamic region (their
    * layout offsets are always 0), so table addresses are the scaled indices */
   s.h = (float *)smem;
   s.pairs = (uint16_t *)smem;
-  s.nb = (uint16_t *)(smem + Lo.nb);
+  s.nb = NBL ? (uint16_t *)(smem + Lo.nb) : gnb;
   s.rowstart = (uint32_t *)(smem + Lo.rowstart);
   s.cursor = (uint32_t *)(smem + Lo.cursor);
   s.cw = (uint32_t *)(smem + Lo.cw);
@@ -221,6 +224,20 @@ __device__ __forceinline__ Lds lds_view(unsigned char *smem, const LdsLayout &Lo
   s.redS = (uint32_t *)(smem + Lo.red + 16 * W + 3 * W * sizeof(BestKey));
   s.ushare = (int *)(smem + Lo.misc) + 2;
   return s;
+}
+
+/* Bins array access: LDS, or (NBL false) the workgroup's global slice.
+ * The global slice is written and read only by the workgroup's own waves,
+ * and every write is followed by a barrier (a workgroup-scope release and
+ * acquire) before another wave reads it, so plain loads and stores are
+ * ordered by the memory model; volatile ones would wait for each load. */
+template <bool NBL>
+__device__ __forceinline__ uint32_t nb_ld(const Lds &S, uint32_t i) {
+  return S.nb[i];
+}
+template <bool NBL>
+__device__ __forceinline__ void nb_st(const Lds &S, uint32_t i, uint16_t v) {
+  S.nb[i] = v;
 }
 
 /* Order-preserving map between floats and ints (an involution), so that an
@@ -324,7 +341,7 @@ __device__ __forceinline__ void shuffle_visit(const Shuffle &s, uint32_t *wsum, 
  * screen-table region, reloaded afterwards when HLDS; or a per-workgroup
  * global buffer when the LDS cannot hold it); the pair's column b is the
  * position's own bin, read again from bin2 when the pairs are sorted. */
-template <int NT, bool EXPLICIT, bool HLDS, bool PL>
+template <int NT, bool EXPLICIT, bool HLDS, bool PL, bool NBL = true>
 __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
                                            const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
                                            uint16_t *gpairs) {
@@ -332,7 +349,7 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   const uint32_t T1 = J.T1, T2 = J.T2, T2wp = Lo.T2w_pad;
   uint16_t *lpairs = S.pairs;
-  volatile uint16_t *vpairs = gpairs; /* global staging: bypass the per-CU cache on the way back */
+  uint16_t *vpairs = gpairs; /* global staging, read back by other waves after a barrier */
   for (uint32_t t = tid; t <= T1; t += NT) S.rowstart[t] = 0;
   for (uint32_t t = tid; t < W * T2wp; t += NT) S.cw[t] = 0;
   if (tid == 0) *S.ushare = fkey(__builtin_inff()); /* published after the build's barriers */
@@ -340,7 +357,7 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
    * kU positions in flight per thread) */
   constexpr uint32_t kU = 16;
   if (!EXPLICIT) {
-    shuffle_visit<NT>(s1, S.redS, [&](uint32_t k, uint32_t m) { S.nb[m] = (uint16_t)J.bin1[k]; });
+    shuffle_visit<NT>(s1, S.redS, [&](uint32_t k, uint32_t m) { nb_st<NBL>(S, m, (uint16_t)J.bin1[k]); });
   } else {
     for (uint32_t k0 = tid; k0 < J.N1; k0 += NT * kU) {
       uint32_t m[kU], bn[kU];
@@ -358,24 +375,25 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
       for (uint32_t u = 0; u < kU; ++u) {
         if (k0 + u * NT < J.N1) {
           if (m[u] < J.N1)
-            S.nb[m[u]] = (uint16_t)bn[u];
+            nb_st<NBL>(S, m[u], (uint16_t)bn[u]);
           else
             atomicOr(err, 1);
         }
       }
     }
   }
+  if (!NBL) __threadfence_block();
   __syncthreads();
   /* B1: (a, b) per list2 position, counted per row a */
   auto pair_of = [&](uint32_t e, uint32_t q, uint32_t b, int32_t jm) {
     uint32_t a = kBinNone;
     if (!J.dup) {
-      if (jm >= 0) a = S.nb[jm];
+      if (jm >= 0) a = nb_ld<NBL>(S, (uint32_t)jm);
     } else {
       /* intersect_genes collapses duplicate ids of genes1: the entry is in
        * S1'(t) as soon as any list1 entry with its id is */
       for (uint32_t t = J.join_off[q]; t < J.join_off[q + 1]; ++t) {
-        uint32_t v = S.nb[J.join_pos[t]];
+        uint32_t v = nb_ld<NBL>(S, J.join_pos[t]);
         a = v < a ? v : a;
       }
     }
@@ -445,10 +463,11 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
     for (uint32_t u = 0; u < kU; ++u) {
       if (a[u] != kBinNone) {
         const uint32_t pos = atomicAdd(&S.cursor[a[u]], 1u);
-        S.nb[pos] = b[u];
+        nb_st<NBL>(S, pos, b[u]);
       }
     }
   }
+  if (!NBL) __threadfence_block();
   __syncthreads();
   if (HLDS && PL) {
     /* the pairs used the screen-table region: put the table back */
@@ -461,13 +480,14 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
  * wave's column counts (four bins in flight per lane).  No wait at the end:
  * the wave's later LDS reads of C are processed after these updates (the
  * LDS executes one wave's operations in order). */
+template <bool NBL = true>
 __device__ __forceinline__ void rows_update(const Lds &S, uint32_t *C, uint32_t ia, uint32_t ib, bool add) {
   const uint32_t lane = lane_id();
   const uint32_t e1 = S.rowstart[ib];
   for (uint32_t e = S.rowstart[ia] + lane; e < e1; e += 256) {
     uint32_t b[4];
 #pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) b[u] = S.nb[min(e + 64u * u, e1 - 1u)];
+    for (uint32_t u = 0; u < 4; ++u) b[u] = nb_ld<NBL>(S, min(e + 64u * u, e1 - 1u));
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) {
       if (e + 64u * u < e1) {
@@ -487,12 +507,14 @@ __device__ __forceinline__ void rows_update(const Lds &S, uint32_t *C, uint32_t 
  * adds without waiting for a load. */
 constexpr uint32_t kPreBins = 5;
 
+template <bool NBL = true>
 __device__ __forceinline__ void prefetch_bins(const Lds &S, uint32_t e0, uint32_t e1, uint32_t *pb) {
   const uint32_t lane = lane_id(), last = e1 > e0 ? e1 - 1u : 0u;
 #pragma unroll
-  for (uint32_t u = 0; u < kPreBins; ++u) pb[u] = S.nb[min(e0 + lane + 64u * u, last)];
+  for (uint32_t u = 0; u < kPreBins; ++u) pb[u] = nb_ld<NBL>(S, min(e0 + lane + 64u * u, last));
 }
 
+template <bool NBL = true>
 __device__ __forceinline__ void add_bins(const Lds &S, uint32_t *C, uint32_t e0, uint32_t e1, const uint32_t *pb) {
   const uint32_t lane = lane_id();
 #pragma unroll
@@ -500,7 +522,7 @@ __device__ __forceinline__ void add_bins(const Lds &S, uint32_t *C, uint32_t e0,
     if (e0 + lane + 64u * u < e1) atomicAdd(&C[pb[u] >> 1], 1u << ((pb[u] & 1u) * 16u));
   /* pairs past the prefetched ones (rows with many pairs) */
   for (uint32_t e = e0 + 64u * kPreBins + lane; e < e1; e += 64u) {
-    const uint32_t b = S.nb[e];
+    const uint32_t b = nb_ld<NBL>(S, e);
     atomicAdd(&C[b >> 1], 1u << ((b & 1u) * 16u));
   }
 }
@@ -1029,7 +1051,7 @@ __device__ __forceinline__ BestKey class_key(double p, const ZeroKey &z) { retur
 /* Exact best among the p == 1 cells (k <= max(0, K+n-N)): larger k, then
  * smaller (i, j).  Only needed when no cell has p < 1; clears the wave's
  * column counts and rebuilds them row by row. */
-template <int NT>
+template <int NT, bool NBL = true>
 __device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout &Lo, const Lds &S, uint32_t *C) {
   constexpr uint32_t W = NT / 64;
   const uint32_t w = wave_id(), lane = lane_id();
@@ -1040,7 +1062,7 @@ __device__ __forceinline__ ZeroKey class1_best(const DevJob &J, const LdsLayout 
   wave_sync();
   uint32_t added = 0;
   for (uint32_t i = w; i < T1; i += W) {
-    rows_update(S, C, added, i + 1, true);
+    rows_update<NBL>(S, C, added, i + 1, true);
     added = i + 1;
     const uint32_t K = J.K[i];
     uint32_t run = row_lane_offset(C, wbeg, c2, T2w);
@@ -1073,14 +1095,14 @@ struct NextTask {
  * thread 0 (others: undefined).  Rows are interleaved over the waves: every
  * wave adds every row's pairs to its own running column counts, and scans
  * the rows i with i % W == w. */
-template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS>
+template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS, bool NBL = true>
 __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
                                  const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
                                  WaveStats &st, uint16_t *gpairs, const NextTask *nt = nullptr) {
   constexpr uint32_t W = NT / 64;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   uint64_t tc0 = cycles();
-  build_rows<NT, EXPLICIT, HLDS, PL>(J, Lo, S, i1, i2, s1, s2, err, gpairs);
+  build_rows<NT, EXPLICIT, HLDS, PL, NBL>(J, Lo, S, i1, i2, s1, s2, err, gpairs);
   uint64_t tc1 = cycles();
   st.cyc[0] += tc1 - tc0;
   /* wave 0 claims the workgroup's next task once the build is done: by then
@@ -1107,7 +1129,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     BestKey xb = no_best();
     uint32_t added = 0;
     for (uint32_t i = w; i < T1; i += W) {
-      rows_update(S, C, added, i + 1, true);
+      rows_update<NBL>(S, C, added, i + 1, true);
       added = i + 1;
       const uint32_t K = J.K[i];
       uint32_t run = row_lane_offset(C, wbeg, c2, T2w);
@@ -1165,7 +1187,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
    * of its first pairs are in pb */
   uint32_t pe0 = S.rowstart[0], pe1 = w < T1 ? S.rowstart[w + 1] : pe0;
   uint32_t pb[kPreBins];
-  prefetch_bins(S, pe0, pe1, pb);
+  prefetch_bins<NBL>(S, pe0, pe1, pb);
   float lq[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
   /* the row constants of 64 steps at a time, lane l holding step 64 b + l's:
    * one load per 64 rows instead of a global load latency per row */
@@ -1178,7 +1200,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       Kl = ir < T1 ? J.K[ir] : 0u;
       Al = ir < T1 ? J.A32[ir] : 0.f;
     }
-    add_bins(S, C, pe0, pe1, pb);
+    add_bins<NBL>(S, C, pe0, pe1, pb);
     added = i + 1;
     const uint32_t inext = i + W;
     uint32_t ne1 = 0u;
@@ -1204,7 +1226,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
         /* the next update's pairs: rows [i + 1, inext + 1) */
         pe0 = pe1;
         pe1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)ne1);
-        prefetch_bins(S, pe0, pe1, pb);
+        prefetch_bins<NBL>(S, pe0, pe1, pb);
       }
       run += tot;
       float smin = __builtin_inff();
@@ -1328,9 +1350,9 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       if (S.rowstart[i + 1u] - S.rowstart[0] < S.rowstart[ctop] - S.rowstart[i + 1u]) {
         /* fewer pairs below than between: rebuild the counts from zero */
         for (uint32_t t = lane; t < Lo.T2w_pad; t += 64u) C[t] = 0u;
-        rows_update(S, C, 0u, i + 1u, true);
+        rows_update<NBL>(S, C, 0u, i + 1u, true);
       } else {
-        rows_update(S, C, i + 1u, ctop, false);
+        rows_update<NBL>(S, C, i + 1u, ctop, false);
       }
       ctop = i + 1u;
       st.rows++;
@@ -1425,7 +1447,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     if (S.redB[2 * W + v].ij != kNone && S.redB[2 * W + v].p < 1.0) need1 = false; /* a p < 1 cell */
   }
   if (need1) {
-    const ZeroKey o = class1_best<NT>(J, Lo, S, C);
+    const ZeroKey o = class1_best<NT, NBL>(J, Lo, S, C);
     if (lane == 0) S.redB[W + w] = class_key(1.0, o);
   } else if (lane == 0) {
     S.redB[W + w] = no_best();
@@ -1503,18 +1525,18 @@ __device__ __forceinline__ void write_record(const DevJob &J, const BestKey &b, 
  * permutations from a device counter (zeroed before the launch), so that
  * permutations of uneven cost spread evenly; every workgroup exits once the
  * counter passes n_perm. */
-template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS>
+template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS, bool NBL>
 __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const uint32_t *__restrict__ idx1,
                                                  const uint32_t *__restrict__ idx2, uint64_t n_perm,
                                                  uint64_t perm_base, uint64_t seed, uint32_t job_id,
                                                  int32_t permuted, dto_record *__restrict__ out,
                                                  int *__restrict__ err, uint16_t *__restrict__ ws,
-                                                 uint32_t *__restrict__ counter) {
+                                                 uint16_t *__restrict__ gnb, uint32_t *__restrict__ counter) {
   /* no static LDS here: the dynamic region then starts at address 0 */
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t &s_next = *(uint32_t *)(smem + Lo.misc);
   constexpr int W = NT / 64;
-  const Lds S = lds_view(smem, Lo, W);
+  const Lds S = lds_view<NBL>(smem, Lo, W, NBL ? nullptr : gnb + (uint64_t)blockIdx.x * max(J.N1, J.N2));
   load_job_tables<NT, HLDS>(J, S);
   __syncthreads();
   uint16_t *gpairs = PL ? nullptr : ws + (uint64_t)blockIdx.x * J.N2;
@@ -1536,7 +1558,8 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
       s1 = shuffle_make(seed, job_id, (uint32_t)pid, 1u, J.N1);
       s2 = shuffle_make(seed, job_id, (uint32_t)pid, 2u, J.N2);
     }
-    BestKey b = optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS>(J, Lo, S, i1, i2, s1, s2, err, st, gpairs, &nt);
+    BestKey b =
+        optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS, NBL>(J, Lo, S, i1, i2, s1, s2, err, st, gpairs, &nt);
     /* a counter value claimed during this task (optimize_perm claims one
      * every time, after the build): published by its last barrier, and new,
      * so the loop always moves on */
@@ -1585,11 +1608,11 @@ static __global__ void k_reset_counter(uint32_t *counter, int *err) {
  * (one atomic counter) until it is drained; a workgroup reloads the per-job
  * LDS tables only when it moves to another job.  Every workgroup exits once
  * the counter passes n_items. */
-template <int NT, bool HLDS, bool PL, bool REGCOLS>
+template <int NT, bool HLDS, bool PL, bool REGCOLS, bool NBL>
 __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs, const WorkItem *__restrict__ items,
                                               uint32_t n_items, uint64_t seed, uint32_t *__restrict__ counter,
                                               int *__restrict__ err, uint16_t *__restrict__ ws, uint32_t ws_stride,
-                                              uint32_t item_slot) {
+                                              uint16_t *__restrict__ gnb, uint32_t gnb_stride, uint32_t item_slot) {
   /* no static LDS here: the dynamic region then starts at address 0.  The
    * work item slot is the misc slot of the batch's largest layout, which
    * every other job's layout ends before. */
@@ -1611,7 +1634,7 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs,
     }
     const WorkItem wi = items[it];
     const BatchJob &B = jobs[wi.job];
-    const Lds S = lds_view(smem, B.lo, W);
+    const Lds S = lds_view<NBL>(smem, B.lo, W, NBL ? nullptr : gnb + (uint64_t)blockIdx.x * gnb_stride);
     if (wi.job != cur) {
       if (last) flush_stats(*last, st);
       st = {0u, 0u, 0u, {0, 0, 0, 0, 0}, 0u, 0u, 0u, 0u, 0u};
@@ -1624,11 +1647,13 @@ __global__ __launch_bounds__(NT) void k_batch(const BatchJob *__restrict__ jobs,
       BestKey b;
       Shuffle s1, s2;
       if (t == 0) {
-        b = optimize_perm<NT, true, HLDS, false, PL, REGCOLS>(B.J, B.lo, S, nullptr, nullptr, s1, s2, err, st, gpairs);
+        b = optimize_perm<NT, true, HLDS, false, PL, REGCOLS, NBL>(B.J, B.lo, S, nullptr, nullptr, s1, s2, err, st,
+                                                                  gpairs);
       } else {
         s1 = shuffle_make(seed, B.job_id, t - 1, 1u, B.J.N1);
         s2 = shuffle_make(seed, B.job_id, t - 1, 2u, B.J.N2);
-        b = optimize_perm<NT, false, HLDS, false, PL, REGCOLS>(B.J, B.lo, S, nullptr, nullptr, s1, s2, err, st, gpairs);
+        b = optimize_perm<NT, false, HLDS, false, PL, REGCOLS, NBL>(B.J, B.lo, S, nullptr, nullptr, s1, s2, err, st,
+                                                                   gpairs);
       }
       if (threadIdx.x == 0) write_record(B.J, b, t == 0 ? 0 : 1, B.out + t, err);
     }

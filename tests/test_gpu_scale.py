@@ -196,3 +196,27 @@ def test_deferred_job_runs_on_first_use(dto, O, gpu):
     l1, l2 = O.OracleList(ids1, r1), O.OracleList(ids2, r2)
     N = O.population_size(l1, l2)
     assert rec_tuple(job.optimize()) == rec_tuple(O.optimize(l1, l2, N, fast=True))
+
+
+def test_large_job_with_list2_duplicates(dto, O, gpu):
+    """N = 24,000 (screen table in LDS, bins global) with duplicate ids in
+    list 2: the CSR join path, k > K cells (p == 0) and their tie counts,
+    in both modes, against the oracle."""
+    from dual_threshold_optimization_amd import workloads as W
+    n = 24000
+    ids1, r1, ids2, r2, _ = W.planted_lists(n, 700, 2400, 1800, 0.1, seed=91)
+    rng = np.random.default_rng(92)
+    for q in rng.choice(len(ids2) - 1, size=400, replace=False):
+        ids2[q + 1] = ids2[q]
+    N = n + 50
+    job = dto.Job.with_population(ids1, r1, ids2, r2, N)
+    l1, l2 = O.OracleList(ids1, r1), O.OracleList(ids2, r2)
+    P = 2
+    i1, i2 = W.explicit_permutations(P, n, n, 93)
+    ub = O.optimize(l1, l2, N, fast=True)
+    ob = O.optimize_many(l1, l2, N, i1, i2, fast=True)
+    for mode in ("screened", "plain"):
+        res, perm = job.run(P, idx1=i1, idx2=i2, mode=mode)
+        assert rec_tuple(res.unpermuted) == rec_tuple(ub), mode
+        for p in range(P):
+            assert rec_tuple(perm[p]) == rec_tuple(ob[p]), (mode, p)
