@@ -67,7 +67,8 @@ LdsLayout dto::make_layout(const HostJob &h, int W, bool with_h, size_t lds_limi
   L.T2w_pad = (T2w + 4u) & ~3u; /* at least one zero padding word */
   /* the screen table (or, without it, the pair staging) sits at offset 0 so
    * that table reads address it with the scaled index alone */
-  const uint32_t hb = with_h ? 4u * (uint32_t)(h.N + 1) : 0u, pb = 2u * std::max(N2, 1u);
+  /* pairs: u32 (row << 16 | column) where they alias the screen table, else the row alone as a u16 */
+  const uint32_t hb = with_h ? 4u * (uint32_t)(h.N + 1) : 0u, pb = (with_h ? 4u : 2u) * std::max(N2, 1u);
   uint32_t fixed = 0;
   auto region = [&](uint32_t &field, uint32_t bytes) {
     field = fixed;

@@ -336,11 +336,12 @@ __device__ __forceinline__ void shuffle_visit(const Shuffle &s, uint32_t *wsum, 
 }
 
 /* Phases A-B for one permutation: leaves the row-sorted list2 bins in S.nb,
- * row offsets in S.rowstart and zeroed per-wave column counts.  The row a of
- * each list2 position's pair is staged as a u16 in `pairs` (LDS: the
- * screen-table region, reloaded afterwards when HLDS; or a per-workgroup
- * global buffer when the LDS cannot hold it); the pair's column b is the
- * position's own bin, read again from bin2 when the pairs are sorted. */
+ * row offsets in S.rowstart and zeroed per-wave column counts.  Each list2
+ * position's pair is staged in `pairs`: with HLDS in the screen-table region
+ * (reloaded afterwards), which holds the whole pair (a << 16 | b) as a u32;
+ * otherwise (LDS next to the bins, or a per-workgroup global buffer) only
+ * its row a as a u16, and the column b, the position's own bin, is read
+ * again from bin2 when the pairs are sorted. */
 template <int NT, bool EXPLICIT, bool HLDS, bool PL, bool NBL = true>
 __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
                                            const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
@@ -397,15 +398,17 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
         a = v < a ? v : a;
       }
     }
-    uint16_t pr = (uint16_t)kBinNone;
+    uint32_t pr = HLDS ? kNone : kBinNone;
     if (a < T1 && b < T2) {
-      pr = (uint16_t)a;
+      pr = HLDS ? (a << 16) | b : a;
       atomicAdd(&S.rowstart[a], 1u);
     }
-    if (PL)
-      lpairs[e] = pr;
+    if (HLDS)
+      ((uint32_t *)lpairs)[e] = pr;
+    else if (PL)
+      lpairs[e] = (uint16_t)pr;
     else
-      vpairs[e] = pr;
+      vpairs[e] = (uint16_t)pr;
   };
   if (!EXPLICIT) {
     shuffle_visit<NT>(s2, S.redS, [&](uint32_t e, uint32_t q) {
@@ -455,8 +458,14 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
       a[u] = kBinNone;
       b[u] = 0;
       if (e < J.N2) {
-        a[u] = PL ? lpairs[e] : vpairs[e];
-        b[u] = J.bin2[e];
+        if (HLDS) {
+          const uint32_t pr = ((const uint32_t *)lpairs)[e];
+          a[u] = pr == kNone ? kBinNone : pr >> 16;
+          b[u] = (uint16_t)(pr & 0xFFFFu);
+        } else {
+          a[u] = PL ? lpairs[e] : vpairs[e];
+          b[u] = J.bin2[e];
+        }
       }
     }
 #pragma unroll
