@@ -53,7 +53,20 @@ def parse():
     p.add_argument("--only", choices=["all", "headline", "calibrate-copy"], default="all")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_optimize_genome_20k_pmc.txt"))
+    p.add_argument("--sq-file", default=os.path.join(ROOT, "profiles", "k_optimize_genome_20k_sq_counters.txt"))
     return p.parse_args()
+
+
+def read_profile_value(path, key):
+    """A `key = value` figure from a committed rocprofv3 summary (None if absent)."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key) and "=" in line:
+                    return float(line.split("=", 1)[1].split()[0])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def read_traffic(path):
@@ -150,10 +163,19 @@ def main():
 
     bytes_per_launch = (P + 1) * (4 * N1 + 4 * N2 + RECORD_BYTES)
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    # the kernel is bound by VALU issue and latency, not by HBM: VALU instructions
+    # per launch (committed SQ counter summary of the same launch) over the
+    # launch's live duration and the chip's wave64 VALU issue rate
+    valu_insts = read_profile_value(args.sq_file, "valu_insts_per_launch")
+    valu_rate = 256 * 4 * 2.4e9 / 4  # CUs x SIMDs x clock / 4 cycles per wave64 instruction
+    valu_frac = None if valu_insts is None else round(valu_insts / (kern_ms * 1e-3 * valu_rate), 4)
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": read_traffic(args.traffic_file),
+                "valu_issue_frac": valu_frac,
+                "valu_issue_basis": "SQ_INSTS_VALU per launch (profiles/k_optimize_genome_20k_sq_counters.txt) / "
+                                    "(live kernel time x 256 CUs x 4 SIMDs x 2.4 GHz / 4)",
                 "kernel": "k_optimize<512 threads, explicit permutations, H table in LDS, screened, pairs in LDS, "
-                          "column constants in registers>",
+                          "column constants in registers, bins in LDS>",
                 "kernel_ms": round(kern_ms, 4),
                 "algorithmic_bytes_per_launch": bytes_per_launch}
 
