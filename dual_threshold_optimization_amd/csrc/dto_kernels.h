@@ -296,22 +296,26 @@ __device__ __forceinline__ void load_job_tables(const DevJob &J, const Lds &S) {
   }
 }
 
-/* Visits the Philox permutation of one list in rank order, fn(k, idx[k])
- * for every k < n, spread over the workgroup.  The domain [0, 2^b) goes in
- * chunks of NT * 16 consecutive elements, each thread evaluating 16
- * consecutive ones; a workgroup prefix count (wave scan + per-wave totals in
- * wsum[W]) ranks those that fall below n.  Every thread runs every chunk. */
+/* Visits the Philox permutation of one list, idx[k] for every k < n, spread
+ * over the workgroup.  The domain goes in chunks of NT * kShufU consecutive
+ * elements, each thread evaluating kShufU consecutive ones; a workgroup
+ * prefix count (wave scan + per-wave totals in wsum[W]) ranks those that
+ * fall below n.  Every thread runs every chunk and hands fn its kShufU
+ * (rank, value) pairs at once, fn(rank[], value[]), value >= n marking an
+ * element outside [0, n), so that fn can issue all its loads together. */
+constexpr uint32_t kShufU = 16;
 template <int NT, typename Fn>
 __device__ __forceinline__ void shuffle_visit(const Shuffle &s, uint32_t *wsum, Fn fn) {
-  constexpr uint32_t kU = 16, W = NT / 64;
+  constexpr uint32_t W = NT / 64;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   uint32_t carried = 0;
-  for (uint32_t base = 0; base < s.dom; base += NT * kU) {
-    uint32_t v[kU], cnt = 0;
+  for (uint32_t base = 0; base < s.dom; base += NT * kShufU) {
+    uint32_t v[kShufU], rk[kShufU], cnt = 0;
 #pragma unroll
-    for (uint32_t u = 0; u < kU; ++u) {
-      const uint32_t p = base + tid * kU + u;
+    for (uint32_t u = 0; u < kShufU; ++u) {
+      const uint32_t p = base + tid * kShufU + u;
       v[u] = p < s.dom ? feistel_once(s, p) : s.n;
+      rk[u] = cnt; /* rank among this thread's elements below n */
       cnt += v[u] < s.n ? 1u : 0u;
     }
     const uint32_t ex = wave_excl_scan(cnt);
@@ -326,12 +330,8 @@ __device__ __forceinline__ void shuffle_visit(const Shuffle &s, uint32_t *wsum, 
     __syncthreads();
     carried += total;
 #pragma unroll
-    for (uint32_t u = 0; u < kU; ++u) {
-      if (v[u] < s.n) {
-        fn(r, v[u]);
-        ++r;
-      }
-    }
+    for (uint32_t u = 0; u < kShufU; ++u) rk[u] += r;
+    fn(rk, v);
   }
 }
 
@@ -358,7 +358,14 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
    * kU positions in flight per thread) */
   constexpr uint32_t kU = 16;
   if (!EXPLICIT) {
-    shuffle_visit<NT>(s1, S.redS, [&](uint32_t k, uint32_t m) { nb_st<NBL>(S, m, (uint16_t)J.bin1[k]); });
+    shuffle_visit<NT>(s1, S.redS, [&](const uint32_t *k, const uint32_t *m) {
+      uint32_t bn[kShufU];
+#pragma unroll
+      for (uint32_t u = 0; u < kShufU; ++u) bn[u] = m[u] < J.N1 ? J.bin1[k[u]] : 0u;
+#pragma unroll
+      for (uint32_t u = 0; u < kShufU; ++u)
+        if (m[u] < J.N1) nb_st<NBL>(S, m[u], (uint16_t)bn[u]);
+    });
   } else {
     for (uint32_t k0 = tid; k0 < J.N1; k0 += NT * kU) {
       uint32_t m[kU], bn[kU];
@@ -411,8 +418,18 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
       vpairs[e] = (uint16_t)pr;
   };
   if (!EXPLICIT) {
-    shuffle_visit<NT>(s2, S.redS, [&](uint32_t e, uint32_t q) {
-      pair_of(e, q, J.bin2[e], J.dup ? -1 : J.join[q]);
+    shuffle_visit<NT>(s2, S.redS, [&](const uint32_t *e, const uint32_t *q) {
+      uint32_t b[kShufU];
+      int32_t jm[kShufU];
+#pragma unroll
+      for (uint32_t u = 0; u < kShufU; ++u) {
+        const bool in = q[u] < J.N2;
+        b[u] = in ? J.bin2[e[u]] : kBinNone;
+        jm[u] = (in && !J.dup) ? J.join[q[u]] : -1;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kShufU; ++u)
+        if (q[u] < J.N2) pair_of(e[u], q[u], b[u], jm[u]);
     });
   } else {
     for (uint32_t e0 = tid; e0 < J.N2; e0 += NT * kU) {
@@ -1747,7 +1764,11 @@ static __global__ __launch_bounds__(256) void k_shuffle(uint64_t seed, uint32_t 
                                                  uint32_t n, uint32_t *__restrict__ out) {
   __shared__ uint32_t wsum[4];
   const Shuffle s = shuffle_make(seed, job, perm, list, n);
-  shuffle_visit<256>(s, wsum, [&](uint32_t k, uint32_t v) { out[k] = v; });
+  shuffle_visit<256>(s, wsum, [&](const uint32_t *k, const uint32_t *v) {
+#pragma unroll
+    for (uint32_t u = 0; u < kShufU; ++u)
+      if (v[u] < n) out[k[u]] = v[u];
+  });
 }
 
 }  // namespace dto

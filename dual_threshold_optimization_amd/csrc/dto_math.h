@@ -225,34 +225,42 @@ DTO_HD U32x4 philox4x32_10(U32x4 c, uint32_t k0, uint32_t k1) {
 }
 
 /* A keyed permutation of [0, n).  F is an alternating Feistel network over
- * the domain [0, 2^b), b = max(12, ceil(log2 n)) (left half ceil(b/2) bits,
- * right half floor(b/2) bits); each round XORs a keyed mix of one half into
- * the other, so every round, and F, is a bijection of the domain.  The
- * permutation is F's image of the domain in order, keeping the values < n:
- *   idx = [F(p) for p = 0, 1, ..., 2^b - 1 if F(p) < n],
- * which holds every value of [0, n) exactly once; the 12-bit floor keeps
- * short lists well mixed (their order is that of their values' preimages in
- * a 4096-element permutation).  (Every domain element is
- * evaluated once, so a GPU workgroup computes it without divergence, with a
- * prefix count for the positions.)  The 8 round keys come from two
- * Philox4x32-10 blocks keyed by the seed, with (permutation, job, list) as
- * the counter.  The mix uses 24-bit multiplies (full rate on the GPU). */
+ * the mixed-radix domain Z_a x Z_b, a = 2^p, of dom = a b elements: for
+ * n <= 4096, a = b = 64 (dom = 4096); above, p = floor(ceil(log2 n) / 2) and
+ * b = ceil(n / a), so dom exceeds n by less than a (under 1 % at n = 20,000,
+ * where a power-of-two domain wastes 64 %).  Element v is the pair
+ * (L = v mod a, R = v div a); even rounds add a keyed mix of R to L mod a,
+ * odd rounds a keyed mix of L, scaled into [0, b), to R mod b.  Each round,
+ * and F, is a bijection of the domain.  The permutation is F's image of the
+ * domain in order, keeping the values < n:
+ *   idx = [F(v) for v = 0, 1, ..., dom - 1 if F(v) < n],
+ * which holds every value of [0, n) exactly once; the 4096 floor keeps short
+ * lists well mixed (their order is that of their values' preimages in a
+ * 4096-element permutation).  (Every domain element is evaluated once, so a
+ * GPU workgroup computes it without divergence, with a prefix count for the
+ * positions.)  The 8 round keys come from two Philox4x32-10 blocks keyed by
+ * the seed, with (permutation, job, list) as the counter.  The mix and the
+ * scaling use 24-bit multiplies (full rate on the GPU). */
 #define DTO_FEISTEL_ROUNDS 8
 struct Shuffle {
-  uint32_t n, bits_l, bits_r, mask_l, mask_r, dom;
+  uint32_t n, pa, amask, b, dom;
   uint32_t rk[DTO_FEISTEL_ROUNDS];
 };
 
 DTO_HD Shuffle shuffle_make(uint64_t seed, uint32_t job, uint32_t perm, uint32_t list, uint32_t n) {
   Shuffle s;
   s.n = n;
-  uint32_t b = 12;
-  while (b < 31 && (1u << b) < n) ++b;
-  s.bits_l = (b + 1) / 2;
-  s.bits_r = b / 2;
-  s.mask_l = (1u << s.bits_l) - 1;
-  s.mask_r = (1u << s.bits_r) - 1;
-  s.dom = 1u << b;
+  uint32_t bl = 0; /* ceil(log2 n) */
+  while (bl < 31 && (1u << bl) < n) ++bl;
+  if (bl <= 12) {
+    s.pa = 6;
+    s.b = 64;
+  } else {
+    s.pa = bl / 2;
+    s.b = (n + (1u << s.pa) - 1) >> s.pa;
+  }
+  s.amask = (1u << s.pa) - 1;
+  s.dom = s.b << s.pa;
   U32x4 c = {perm, job, list, 0x44544F32u};
   U32x4 a = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
   c.w ^= 0x5A5A5A5Au;
@@ -277,17 +285,21 @@ DTO_HD uint32_t feistel_mix(uint32_t v, uint32_t key) {
   return h ^ (key >> 24);
 }
 
-/* F(p): the Feistel bijection of [0, 2^b). */
+/* F(v): the Feistel bijection of [0, dom). */
 DTO_HD uint32_t feistel_once(const Shuffle &s, uint32_t v) {
-  uint32_t L = v >> s.bits_r, R = v & s.mask_r;
+  uint32_t L = v & s.amask, R = v >> s.pa;
 #pragma unroll
   for (int r = 0; r < DTO_FEISTEL_ROUNDS; ++r) {
-    if ((r & 1) == 0)
-      L ^= feistel_mix(R, s.rk[r]) & s.mask_l;
-    else
-      R ^= feistel_mix(L, s.rk[r]) & s.mask_r;
+    if ((r & 1) == 0) {
+      L = (L + feistel_mix(R, s.rk[r])) & s.amask;
+    } else {
+      /* the mix's low 16 bits scaled into [0, b) (b <= 2^16, so the product
+       * fits; the mask on b, a no-op, lets the GPU use a 24-bit multiply) */
+      R += ((feistel_mix(L, s.rk[r]) & 0xFFFFu) * (s.b & 0xFFFFFFu)) >> 16;
+      R = R >= s.b ? R - s.b : R;
+    }
   }
-  return (L << s.bits_r) | R;
+  return (R << s.pa) | L;
 }
 
 /* The whole permutation (host): out[k] = idx[k], k < n. */

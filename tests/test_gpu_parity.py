@@ -55,7 +55,7 @@ def test_hypergeometric_random_vs_oracle(dto, O, gpu):
 
 
 def test_philox_device_matches_host(dto, gpu):
-    for n in (1, 2, 3, 30, 1000, 20000, 32768):
+    for n in (1, 2, 3, 30, 1000, 4096, 4097, 20000, 32768, 65535):
         for perm in (0, 7):
             host = dto.philox_permutation(99, 3, perm, 1, n)
             dev = np.zeros(n, dtype=np.uint32)

@@ -242,7 +242,7 @@ def test_philox_known_answers(dto):
 
 def test_philox_shuffle_is_a_permutation(dto):
     from dual_threshold_optimization_amd.collections import philox_permutation
-    for n in [0, 1, 2, 3, 5, 64, 1000, 4097, 20000]:
+    for n in [0, 1, 2, 3, 5, 64, 1000, 4096, 4097, 20000, 65535]:
         p = philox_permutation(7, 0, 1, 1, n)
         assert sorted(p.tolist()) == list(range(n))
         assert np.array_equal(p, philox_permutation(7, 0, 1, 1, n))
@@ -255,6 +255,29 @@ def test_philox_shuffle_is_a_permutation(dto):
     firsts = np.array([philox_permutation(s, 0, 0, 1, 10)[0] for s in range(2000)])
     counts = np.bincount(firsts, minlength=10)
     assert counts.min() > 140 and counts.max() < 260
+
+
+def test_philox_shuffle_uniformity(dto):
+    """Position-by-value counts of many shuffles against a uniform shuffle:
+    chi-square of the full table at n = 30 (power-of-two floor domain), and
+    of a 50 x 50 binned table at n = 20,000 (mixed-radix domain, 1 % above n);
+    adjacent positions correlate as a uniform shuffle's do, -1 / (n - 1)."""
+    from dual_threshold_optimization_amd.collections import philox_permutation
+
+    def z_score(n, S, B):
+        M = np.zeros((B, B))
+        for s in range(S):
+            p = philox_permutation(1000 + s, 5, s, 2, n).astype(np.int64)
+            np.add.at(M, ((np.arange(n) * B) // n, (p * B) // n), 1)
+        exp = M.sum() / (B * B)
+        chi = ((M - exp) ** 2 / exp).sum()
+        dof = (B - 1) ** 2
+        return (chi - dof) / np.sqrt(2 * dof)
+    assert abs(z_score(30, 12000, 30)) < 4
+    assert abs(z_score(20000, 120, 50)) < 4
+    adj = [np.corrcoef(p[:-1], p[1:])[0, 1] for p in
+           (philox_permutation(s, 0, s, 1, 1000).astype(np.float64) for s in range(1500))]
+    assert abs(np.mean(adj) + 1 / 999) < 4 * np.std(adj) / np.sqrt(len(adj))
 
 
 # ---- inputs, population rules and errors ---------------------------------------------
