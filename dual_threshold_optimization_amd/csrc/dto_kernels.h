@@ -124,7 +124,36 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) { return wave_incl_scan(v) - v; }
 
-/* Minimum over the wave, returned in every lane. */
+/* Order-preserving map between float bit patterns and ints (an involution,
+ * exact for every non-NaN value), so that wave minima are integer minima:
+ * v_min_i32 takes its DPP operand directly and needs no NaN quieting. */
+__device__ __forceinline__ int ord_key(int b) { return b ^ ((b >> 31) & 0x7FFFFFFF); }
+
+/* Minimum over the wave of each of R values (never NaN), returned in every
+ * lane; the R reductions are interleaved so that their DPP latencies
+ * overlap. */
+template <int R>
+__device__ __forceinline__ void wave_minf_n(float *v) {
+  constexpr int idn = 0x7FFFFFFF; /* identity of the integer minimum */
+  int x[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) x[r] = ord_key(__float_as_int(v[r]));
+#define DTO_MIN_STEP(ctrl, rmask)                                                                                  \
+  _Pragma("unroll") for (int r = 0; r < R; ++r) x[r] =                                                            \
+      min(x[r], __builtin_amdgcn_update_dpp(idn, x[r], ctrl, rmask, 0xf, false));
+  DTO_MIN_STEP(0x111, 0xf) /* row_shr:1 */
+  DTO_MIN_STEP(0x112, 0xf) /* row_shr:2 */
+  DTO_MIN_STEP(0x114, 0xf) /* row_shr:4 */
+  DTO_MIN_STEP(0x118, 0xf) /* row_shr:8 */
+  DTO_MIN_STEP(0x142, 0xa) /* row_bcast:15 */
+  DTO_MIN_STEP(0x143, 0xc) /* row_bcast:31 */
+#undef DTO_MIN_STEP
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = __int_as_float(ord_key(__builtin_amdgcn_readlane(x[r], 63)));
+}
+
+/* Minimum over the wave, returned in every lane (fminf: a NaN lane is
+ * ignored). */
 __device__ __forceinline__ float wave_minf(float v) {
   const int inf = __float_as_int(__builtin_inff());
   int x = __float_as_int(v);
@@ -135,28 +164,6 @@ __device__ __forceinline__ float wave_minf(float v) {
   x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(inf, x, 0x142, 0xa, 0xf, false))));
   x = __float_as_int(fminf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(inf, x, 0x143, 0xc, 0xf, false))));
   return __int_as_float(__builtin_amdgcn_readlane(x, 63));
-}
-
-/* Minimum over the wave of each of R values, returned in every lane; the R
- * reductions are interleaved so that their DPP latencies overlap. */
-template <int R>
-__device__ __forceinline__ void wave_minf_n(float *v) {
-  const int inf = __float_as_int(__builtin_inff());
-  int x[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) x[r] = __float_as_int(v[r]);
-#define DTO_MIN_STEP(ctrl, rmask)                                                                                  \
-  _Pragma("unroll") for (int r = 0; r < R; ++r) x[r] = __float_as_int(                                            \
-      fminf(__int_as_float(x[r]), __int_as_float(__builtin_amdgcn_update_dpp(inf, x[r], ctrl, rmask, 0xf, false))));
-  DTO_MIN_STEP(0x111, 0xf)
-  DTO_MIN_STEP(0x112, 0xf)
-  DTO_MIN_STEP(0x114, 0xf)
-  DTO_MIN_STEP(0x118, 0xf)
-  DTO_MIN_STEP(0x142, 0xa)
-  DTO_MIN_STEP(0x143, 0xc)
-#undef DTO_MIN_STEP
-#pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = __int_as_float(__builtin_amdgcn_readlane(x[r], 63));
 }
 
 /* Exclusive scan of data[0..len) in LDS, total written to data[len]. */
@@ -958,7 +965,26 @@ __device__ __forceinline__ float pack_slack(float v) { return fabsf(v) * 0x1p-18
 struct ColRegs {
   uint32_t n4[kSegCells]; /* 4 n_j */
   float b[kSegCells];
+  uint32_t wa[kSegWords]; /* LDS byte addresses of the lane's count words */
 };
+
+/* LDS byte address of a pointer into the dynamic region. */
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+
+/* read_segment for a row held in registers (REGCOLS): the lane's word
+ * addresses are fixed for the job, and words past the lane's columns point
+ * at the wave's zero padding word, so the loads need neither index
+ * arithmetic nor masks. */
+__device__ __forceinline__ uint32_t read_segment_regs(const ColRegs &cr, uint32_t *wv) {
+#pragma unroll
+  for (uint32_t u = 0; u < kSegWords; ++u) wv[u] = *(const __attribute__((address_space(3))) uint32_t *)cr.wa[u];
+  uint32_t tot = 0;
+#pragma unroll
+  for (uint32_t u = 0; u < kSegWords; ++u) tot += (wv[u] & 0xFFFFu) + (wv[u] >> 16);
+  return tot;
+}
 
 /* Reads a segment's count words (all loads issued together) and returns
  * the lane's count total.  Indices are clamped to the wave's last padding
@@ -994,9 +1020,9 @@ __device__ __forceinline__ uint32_t read_segment(const uint32_t *C, uint32_t w0,
  * that can wrap for cells of the other classes saturate at 0. */
 template <bool HLDS, bool REGCOLS, bool ZC = true>
 __device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, const char *Hb, const uint32_t *wv,
-                                             uint32_t nseg, uint32_t w0, uint32_t run, uint32_t K, float A,
-                                             const ColRegs &cr, uint32_t *kk, uint32_t *nn4, float *xx) {
-  const uint32_t T2 = J.T2, K4 = K << 2, NK4 = (J.N - K) << 2;
+                                             uint32_t nseg, uint32_t w0, uint32_t run, uint32_t K, uint32_t NK4,
+                                             float A, const ColRegs &cr, uint32_t *kk, uint32_t *nn4, float *xx) {
+  const uint32_t T2 = J.T2, K4 = K << 2;
   float bc[kSegCells], h0[kSegCells], h1[kSegCells], h2[kSegCells], h3[kSegCells];
   uint32_t d2[kSegCells];
   /* indices and table loads of every cell first (one wait for all) */
@@ -1196,6 +1222,11 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       cr.n4[c] = S.coln[jc] << 2;
       cr.b[c] = valid ? S.colb[jc] : kNoCell;
     }
+#pragma unroll
+    for (uint32_t u = 0; u < kSegWords; ++u) {
+      const uint32_t last = Lo.T2w_pad - 1u;
+      cr.wa[u] = lds_addr(C + (u < c2 ? min(wbeg + u, last) : last));
+    }
   }
   const float margin = J.margin;
   const float uthr = -745.2f - margin; /* below: the first term may underflow */
@@ -1217,13 +1248,14 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   float lq[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
   /* the row constants of 64 steps at a time, lane l holding step 64 b + l's:
    * one load per 64 rows instead of a global load latency per row */
-  uint32_t Kl = 0u;
+  uint32_t Kl = 0u, NKl = 0u; /* K_i and 4 (N - K_i) */
   float Al = 0.f;
   uint32_t pfacc = 0u; /* keeps the prefetch loads */
   for (uint32_t i = w, q = 0; i < T1; i += W, ++q) {
     if ((q & 63u) == 0u) {
       const uint32_t ir = i + lane * W;
       Kl = ir < T1 ? J.K[ir] : 0u;
+      NKl = (J.N - Kl) << 2;
       Al = ir < T1 ? J.A32[ir] : 0.f;
     }
     add_bins<NBL>(S, C, pe0, pe1, pb);
@@ -1232,6 +1264,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     uint32_t ne1 = 0u;
     if (inext < T1) ne1 = S.rowstart[inext + 1];
     const uint32_t K = (uint32_t)__builtin_amdgcn_readlane((int)Kl, (int)(q & 63u));
+    const uint32_t NK4 = (uint32_t)__builtin_amdgcn_readlane((int)NKl, (int)(q & 63u));
     const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Al), (int)(q & 63u)));
     float lmin = __builtin_inff(); /* packed (value, cell) */
     uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
@@ -1240,14 +1273,14 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     for (uint32_t t0 = 0; REGCOLS ? t0 == 0u : t0 < c2; t0 += kSegWords) {
       const uint32_t nseg = REGCOLS ? c2 : (c2 - t0 < kSegWords ? c2 - t0 : kSegWords);
       uint32_t wv[kSegWords];
-      const uint32_t tot = read_segment(C, wbeg + t0, nseg, Lo.T2w_pad - 1u, wv);
+      const uint32_t tot = REGCOLS ? read_segment_regs(cr, wv) : read_segment(C, wbeg + t0, nseg, Lo.T2w_pad - 1u, wv);
       if (REGCOLS) run = wave_excl_scan(tot);
       uint32_t kk[kSegCells], nn4[kSegCells];
       float xx[kSegCells];
       if (J.kle)
-        segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+        segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
       else
-        segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+        segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
       if (t0 == 0u && inext < T1) {
         /* the next update's pairs: rows [i + 1, inext + 1) */
         pe0 = pe1;
@@ -1360,6 +1393,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     const bool have = b0 + lane < nown;
     const float rm = have ? S.rowmin[top - lane * W] : __builtin_inff();
     const uint32_t Kl = have ? J.K[top - lane * W] : 0u;
+    const uint32_t NKl = (J.N - Kl) << 2;
     const float Al = have ? J.A32[top - lane * W] : 0.f;
     uint64_t done = 0;
 #pragma unroll 1
@@ -1383,6 +1417,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       ctop = i + 1u;
       st.rows++;
       const uint32_t K = (uint32_t)__builtin_amdgcn_readlane((int)Kl, (int)l);
+      const uint32_t NK4 = (uint32_t)__builtin_amdgcn_readlane((int)NKl, (int)l);
       const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Al), (int)l));
       /* a row whose survivors overflow the list is flushed and rescanned from
        * the cell that did not fit (cells are visited in a fixed wave-uniform
@@ -1396,14 +1431,15 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
         for (uint32_t t0 = 0; (REGCOLS ? t0 == 0u : t0 < c2) && !again; t0 += kSegWords) {
           const uint32_t nseg = REGCOLS ? c2 : (c2 - t0 < kSegWords ? c2 - t0 : kSegWords);
           uint32_t wv[kSegWords];
-          const uint32_t tot = read_segment(C, wbeg + t0, nseg, Lo.T2w_pad - 1u, wv);
+          const uint32_t tot =
+              REGCOLS ? read_segment_regs(cr, wv) : read_segment(C, wbeg + t0, nseg, Lo.T2w_pad - 1u, wv);
           if (REGCOLS) run = wave_excl_scan(tot);
           uint32_t kk[kSegCells], nn4[kSegCells];
           float xx[kSegCells];
           if (J.kle)
-            segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+            segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
           else
-            segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, A, cr, kk, nn4, xx);
+            segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
           run += tot;
           /* cheap first test: the first-term bound alone; cells whose bound
            * is below -708 (possibly subnormal or zero sums) always pass */

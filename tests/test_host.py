@@ -272,12 +272,22 @@ def test_philox_shuffle_uniformity(dto):
         exp = M.sum() / (B * B)
         chi = ((M - exp) ** 2 / exp).sum()
         dof = (B - 1) ** 2
-        return (chi - dof) / np.sqrt(2 * dof)
+        # each cell is a sum of S hypergeometric draws (n/B positions, n/B
+        # values), so the statistic's mean is dof * n / (n - 1), not dof
+        return (chi - dof * n / (n - 1)) / np.sqrt(2 * dof)
     assert abs(z_score(30, 12000, 30)) < 4
     assert abs(z_score(20000, 120, 50)) < 4
     adj = [np.corrcoef(p[:-1], p[1:])[0, 1] for p in
            (philox_permutation(s, 0, s, 1, 1000).astype(np.float64) for s in range(1500))]
     assert abs(np.mean(adj) + 1 / 999) < 4 * np.std(adj) / np.sqrt(len(adj))
+    # the joint law at n = 4: all 24 orders equally frequent (multinomial chi-square, 23 dof)
+    counts = {}
+    for s in range(24000):
+        key = tuple(philox_permutation(5000 + s, 1, s, 1, 4).tolist())
+        counts[key] = counts.get(key, 0) + 1
+    assert len(counts) == 24
+    chi = sum((c - 1000) ** 2 / 1000 for c in counts.values())
+    assert abs(chi - 23) / np.sqrt(46) < 4
 
 
 # ---- inputs, population rules and errors ---------------------------------------------
