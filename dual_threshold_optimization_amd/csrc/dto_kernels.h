@@ -1483,10 +1483,22 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   st.cyc[3] += tc4 - tc3;
   if (nt && w == 0u && nt->idx1 && nt->idx2 && pnext < nt->n_perm) {
     /* the next task's index arrays, late in this task so that they are
-     * still in the cache when its build reads them */
-    const volatile uint32_t *p1 = nt->idx1 + (uint64_t)pnext * J.N1, *p2 = nt->idx2 + (uint64_t)pnext * J.N2;
-    for (uint32_t t = lane; t < (J.N1 + 31u) / 32u; t += 64u) pfacc += p1[32u * t];
-    for (uint32_t t = lane; t < (J.N2 + 31u) / 32u; t += 64u) pfacc += p2[32u * t];
+     * still in the cache when its build reads them: one load per 128-byte
+     * line, kPf lines per lane issued together before a single wait (the
+     * sum keeps the loads) */
+    constexpr uint32_t kPf = 16;
+    const uint32_t *p1 = nt->idx1 + (uint64_t)pnext * J.N1, *p2 = nt->idx2 + (uint64_t)pnext * J.N2;
+    const uint32_t l1 = (J.N1 + 31u) / 32u, l2 = (J.N2 + 31u) / 32u;
+    for (uint32_t t0 = 0; t0 < l1 + l2; t0 += 64u * kPf) {
+      uint32_t v[kPf];
+#pragma unroll
+      for (uint32_t u = 0; u < kPf; ++u) {
+        const uint32_t t = t0 + u * 64u + lane;
+        v[u] = t < l1 ? p1[32u * t] : (t < l1 + l2 ? p2[32u * (t - l1)] : 0u);
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kPf; ++u) pfacc += v[u];
+    }
   }
   /* tier 3: exact FP64 for cells whose bracket reaches the best bound this
    * wave knows of (its own, and what the other waves have shared so far: no
