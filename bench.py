@@ -372,7 +372,7 @@ def serving(dto, W, np, args):
     t = time.perf_counter()
     jobs = _make_jobs(dto, sets, which, threads)
     log("serving: %d deferred jobs built (%.1fs)" % (R, time.perf_counter() - t))
-    sv = dto.Server(hbm_budget_bytes=8 << 30, perms_per_slice=256, seed=5)
+    sv = dto.Server(hbm_budget_bytes=8 << 30, perms_per_slice=1024, seed=5)
     lat, done, i = [], 0, 0
     t0 = time.perf_counter()
     while done < R:

@@ -677,7 +677,7 @@ int dto_server_create(const dto_server_config *cfg, dto_server **out) {
   dto_server *sv = new dto_server();
   sv->cfg = *cfg;
   if (!sv->cfg.max_active_jobs) sv->cfg.max_active_jobs = 256;
-  if (!sv->cfg.perms_per_slice) sv->cfg.perms_per_slice = 256;
+  if (!sv->cfg.perms_per_slice) sv->cfg.perms_per_slice = 1024;
   if (!sv->cfg.hbm_budget_bytes) sv->cfg.hbm_budget_bytes = 8ull << 30;
   DevProps pr;
   int rc = current_device(&sv->device);

@@ -90,7 +90,7 @@ class Batch:
 class Server:
     """Continuous-batching request loop (see dto_server_* in include/dto.h)."""
 
-    def __init__(self, hbm_budget_bytes=8 << 30, max_active_jobs=256, perms_per_slice=256, seed=0):
+    def __init__(self, hbm_budget_bytes=8 << 30, max_active_jobs=256, perms_per_slice=1024, seed=0):
         cfg = _lib.ServerConfig(hbm_budget_bytes, max_active_jobs, perms_per_slice, seed)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().dto_server_create(ctypes.byref(cfg), ctypes.byref(h)))

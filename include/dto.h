@@ -221,9 +221,9 @@ int dto_batch_autotune(dto_batch *b, int *waves);
 /* ---- serving (continuous batching) --------------------------------------- */
 typedef struct dto_server dto_server;
 typedef struct {
-  uint64_t hbm_budget_bytes; /* admission limit on resident job state */
-  uint32_t max_active_jobs;
-  uint32_t perms_per_slice;  /* permutations per job per launch slice */
+  uint64_t hbm_budget_bytes; /* admission limit on resident job state (0: 8 GiB) */
+  uint32_t max_active_jobs;  /* requests with tasks in flight at once (0: 256) */
+  uint32_t perms_per_slice;  /* permutations per job per launch slice (0: 1024) */
   uint64_t seed;
 } dto_server_config;
 int dto_server_create(const dto_server_config *cfg, dto_server **out);
