@@ -1173,7 +1173,11 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
 
   const uint32_t T1 = J.T1, T2 = J.T2, T2w = J.T2w;
   uint32_t *C = S.cw + w * Lo.T2w_pad;
-  const uint32_t c2 = (T2w + 63) / 64;
+  /* count words per lane; wide grids (several segments per row) take an odd
+   * number, so that the lanes' words, and their columns' constants, fall in
+   * different LDS banks (an even stride puts 64 lanes on 64 / 2^t banks).
+   * Lanes past the grid read the zero padding word and hold no cell. */
+  const uint32_t c2 = REGCOLS ? (T2w + 63) / 64 : ((T2w + 63) / 64) | 1u;
   const uint32_t wbeg = lane * c2;
   const char *Hb = HLDS ? (const char *)S.h : (const char *)J.H32;
 
