@@ -1065,6 +1065,49 @@ __device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, cons
   __builtin_amdgcn_sched_barrier(0);
 }
 
+/* flush_first's tier-1 test, run on a segment's first-term survivors where
+ * they sit in registers: a segment with more survivors than the first-term
+ * list holds is tested here rather than through passes of flush and rescan.
+ * The cells are row i's (constants K, and x as the screen computed it, which
+ * is the value flush_first rebuilds); survivors flagged in surv, column of
+ * cell c is j0 + c.  Tier-1 survivors go to the tier-2 list. */
+template <int NT>
+__device__ __forceinline__ void tier1_segment(const DevJob &J, const Lds &S, uint32_t surv, const uint32_t *kk,
+                                              const uint32_t *nn4, const float *xx, uint32_t K, uint32_t i,
+                                              uint32_t j0, uint2 *cand, uint32_t &nc, uint4 *c3, uint32_t &n3,
+                                              BestKey &xb, float &Uw, float &U2w, WaveStats &st) {
+  const uint32_t N = J.N;
+#pragma unroll
+  for (uint32_t c = 0; c < kSegCells; ++c) {
+    bool f = false;
+    const uint32_t k = kk[c], n = nn4[c] >> 2;
+    if (((surv >> c) & 1u) != 0u && k <= min(K, n)) {
+      const float x = xx[c];
+      if (!(x + J.margin < -745.2f && exact_zero(J, K, n, k, x))) {
+        float L = x - J.margin;
+        if (L >= -708.f) {
+          L += tail_lower_log(N, K, n, k);
+          f = L <= Uw;
+        } else {
+          f = true; /* possibly subnormal: no usable lower bound */
+        }
+      }
+    }
+    const uint64_t m = __ballot(f);
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    if (cnt) {
+      if (nc + cnt > (uint32_t)kCandCap) {
+        uint64_t tf = cycles();
+        flush_candidates<NT>(J, S, cand, nc, c3, n3, xb, Uw, U2w, st);
+        st.cyc[3] += cycles() - tf;
+      }
+      if (f) cand[nc + mask_lt(m)] = make_uint2((i << 16) | (j0 + c), k);
+      nc += cnt;
+      st.push1 += cnt;
+    }
+  }
+}
+
 /* The best of a class of cells that share one p (p == 0, or p == 1), in
  * optimize's order at equal p (optimize_main.rs:93-116): larger k, then
  * smaller (i, j); n counts the class's cells, nk those at the best k. */
@@ -1456,8 +1499,11 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
           uint32_t stot = 0;
 #pragma unroll
           for (uint32_t c = 0; c < kSegCells; ++c) stot += (uint32_t)__popcll(__ballot(((surv >> c) & 1u) != 0u));
-          /* make room first; only a segment with more survivors than the list
-           * holds goes through the flush-and-resume path below */
+          if (stot > (uint32_t)kCand0Cap) {
+            tier1_segment<NT>(J, S, surv, kk, nn4, xx, K, i, 2u * (wbeg + t0), cand, nc, c3, n3, xb, Uw, U2w, st);
+            continue;
+          }
+          /* make room first: the segment then fits the list */
           if (n0 + stot > (uint32_t)kCand0Cap)
             flush_first<NT, HLDS>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st, top, Kl, Al);
 #pragma unroll
