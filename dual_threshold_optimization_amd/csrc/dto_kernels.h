@@ -675,8 +675,8 @@ __device__ __forceinline__ double first_exponent(const DevJob &J, uint32_t K, ui
 
 struct WaveStats {
   uint32_t cells, terms, rows;
-  uint64_t cyc[5]; /* build, forward sweep, reverse sweep, tier-1 flushes, exact */
-  uint32_t push1, push2, flushes; /* tier-1 pushes, tier-2 survivors, tier-1 list flushes */
+  uint64_t cyc[5]; /* build, forward sweep, reverse sweep, tier-2 brackets (flush_candidates), exact */
+  uint32_t push1, push2, flushes; /* tier-1 survivors, tier-2 survivors, flush_candidates calls */
   uint32_t gap;                    /* sum of (first upper bound - log of the minimum), milli-nats */
   uint32_t cheap;                  /* cells passing the first-term test of the reverse sweep (per lane) */
 };
