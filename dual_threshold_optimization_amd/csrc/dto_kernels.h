@@ -1466,61 +1466,47 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       const uint32_t K = (uint32_t)__builtin_amdgcn_readlane((int)Kl, (int)l);
       const uint32_t NK4 = (uint32_t)__builtin_amdgcn_readlane((int)NKl, (int)l);
       const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Al), (int)l));
-      /* a row whose survivors overflow the list is flushed and rescanned from
-       * the cell that did not fit (cells are visited in a fixed wave-uniform
-       * order, ordinal 2 t0 + c), so every pass makes progress */
-      uint32_t resume = 0;
+      uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
 #pragma unroll 1
-      for (bool again = true; again;) {
-        again = false;
-        uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
-#pragma unroll 1
-        for (uint32_t t0 = 0; (REGCOLS ? t0 == 0u : t0 < c2) && !again; t0 += kSegWords) {
-          const uint32_t nseg = REGCOLS ? c2 : (c2 - t0 < kSegWords ? c2 - t0 : kSegWords);
-          uint32_t wv[kSegWords];
-          const uint32_t tot =
-              REGCOLS ? read_segment_regs(cr, wv) : read_segment(C, wbeg + t0, nseg, Lo.T2w_pad - 1u, wv);
-          if (REGCOLS) run = wave_excl_scan(tot);
-          uint32_t kk[kSegCells], nn4[kSegCells];
-          float xx[kSegCells];
-          if (J.kle)
-            segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
-          else
-            segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
-          run += tot;
-          /* cheap first test: the first-term bound alone; cells whose bound
-           * is below -708 (possibly subnormal or zero sums) always pass */
-          const float thr = fmaxf(Uw + margin, -708.f + margin);
-          uint32_t surv = 0;
+      for (uint32_t t0 = 0; REGCOLS ? t0 == 0u : t0 < c2; t0 += kSegWords) {
+        const uint32_t nseg = REGCOLS ? c2 : (c2 - t0 < kSegWords ? c2 - t0 : kSegWords);
+        uint32_t wv[kSegWords];
+        const uint32_t tot =
+            REGCOLS ? read_segment_regs(cr, wv) : read_segment(C, wbeg + t0, nseg, Lo.T2w_pad - 1u, wv);
+        if (REGCOLS) run = wave_excl_scan(tot);
+        uint32_t kk[kSegCells], nn4[kSegCells];
+        float xx[kSegCells];
+        if (J.kle)
+          segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
+        else
+          segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
+        run += tot;
+        /* cheap first test: the first-term bound alone; cells whose bound
+         * is below -708 (possibly subnormal or zero sums) always pass */
+        const float thr = fmaxf(Uw + margin, -708.f + margin);
+        uint32_t surv = 0;
 #pragma unroll
-          for (uint32_t c = 0; c < kSegCells; ++c) surv |= (xx[c] <= thr && 2u * t0 + c >= resume) ? 1u << c : 0u;
-          if (!__ballot(surv != 0u)) continue;
-          st.cheap += (uint32_t)__popc(surv);
-          uint32_t stot = 0;
+        for (uint32_t c = 0; c < kSegCells; ++c) surv |= xx[c] <= thr ? 1u << c : 0u;
+        if (!__ballot(surv != 0u)) continue;
+        st.cheap += (uint32_t)__popc(surv);
+        uint32_t stot = 0;
 #pragma unroll
-          for (uint32_t c = 0; c < kSegCells; ++c) stot += (uint32_t)__popcll(__ballot(((surv >> c) & 1u) != 0u));
-          if (stot > (uint32_t)kCand0Cap) {
-            tier1_segment<NT>(J, S, surv, kk, nn4, xx, K, i, 2u * (wbeg + t0), cand, nc, c3, n3, xb, Uw, U2w, st);
-            continue;
-          }
-          /* make room first: the segment then fits the list */
-          if (n0 + stot > (uint32_t)kCand0Cap)
-            flush_first<NT, HLDS>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st, top, Kl, Al);
-#pragma unroll
-          for (uint32_t c = 0; c < kSegCells; ++c) {
-            const bool f = ((surv >> c) & 1u) != 0u && !again;
-            const uint64_t m = __ballot(f);
-            const uint32_t cnt = (uint32_t)__popcll(m);
-            if (n0 + cnt > (uint32_t)kCand0Cap) {
-              again = true;
-              resume = 2u * t0 + c;
-            } else if (cnt) {
-              if (f) cand0[n0 + mask_lt(m)] = make_uint2((i << 16) | (2u * (wbeg + t0) + c), kk[c]);
-              n0 += cnt;
-            }
-          }
+        for (uint32_t c = 0; c < kSegCells; ++c) stot += (uint32_t)__popcll(__ballot(((surv >> c) & 1u) != 0u));
+        if (stot > (uint32_t)kCand0Cap) {
+          /* more survivors than the list holds: tier 1 where they are */
+          tier1_segment<NT>(J, S, surv, kk, nn4, xx, K, i, 2u * (wbeg + t0), cand, nc, c3, n3, xb, Uw, U2w, st);
+          continue;
         }
-        /* again: the next pass starts by flushing the full list */
+        /* make room first: then the segment's survivors fit the list */
+        if (n0 + stot > (uint32_t)kCand0Cap)
+          flush_first<NT, HLDS>(J, S, Hb, cand0, n0, cand, nc, c3, n3, xb, Uw, U2w, st, top, Kl, Al);
+#pragma unroll
+        for (uint32_t c = 0; c < kSegCells; ++c) {
+          const bool f = ((surv >> c) & 1u) != 0u;
+          const uint64_t m = __ballot(f);
+          if (f) cand0[n0 + mask_lt(m)] = make_uint2((i << 16) | (2u * (wbeg + t0) + c), kk[c]);
+          n0 += (uint32_t)__popcll(m);
+        }
       }
     }
     /* the list holds rows of this batch only */
