@@ -1189,8 +1189,11 @@ struct NextTask {
 /* One permutation: overlap grid -> best cell.  Returns the best key in
  * thread 0 (others: undefined).  Rows are interleaved over the waves: every
  * wave adds every row's pairs to its own running column counts, and scans
- * the rows i with i % W == w. */
-template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS, bool NBL = true>
+ * the rows i with i % W == w.  ZCM, the screen's k > K test (cells with an
+ * empty tail): 1 in, 0 left out (the job cannot have such cells,
+ * DevJob.kle), 2 chosen by J.kle at run time (the batch kernel, whose jobs
+ * differ). */
+template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS, bool NBL = true, int ZCM = 2>
 __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayout &Lo, const Lds &S, const uint32_t *i1,
                                  const uint32_t *i2, const Shuffle &s1, const Shuffle &s2, int *err,
                                  WaveStats &st, uint16_t *gpairs, const NextTask *nt = nullptr) {
@@ -1324,7 +1327,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       if (REGCOLS) run = wave_excl_scan(tot);
       uint32_t kk[kSegCells], nn4[kSegCells];
       float xx[kSegCells];
-      if (J.kle)
+      if (ZCM == 2 ? J.kle != 0u : ZCM == 0)
         segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
       else
         segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
@@ -1476,7 +1479,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
         if (REGCOLS) run = wave_excl_scan(tot);
         uint32_t kk[kSegCells], nn4[kSegCells];
         float xx[kSegCells];
-        if (J.kle)
+        if (ZCM == 2 ? J.kle != 0u : ZCM == 0)
           segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
         else
           segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
@@ -1634,8 +1637,9 @@ __device__ __forceinline__ void write_record(const DevJob &J, const BestKey &b, 
  * workgroup pair staging (N2 u16 each) when !PL.  Workgroups take
  * permutations from a device counter (zeroed before the launch), so that
  * permutations of uneven cost spread evenly; every workgroup exits once the
- * counter passes n_perm. */
-template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS, bool NBL>
+ * counter passes n_perm.  ZC false: the job cannot have cells with k > K
+ * (DevJob.kle), and the screen leaves that test out. */
+template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS, bool NBL, bool ZC>
 __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const uint32_t *__restrict__ idx1,
                                                  const uint32_t *__restrict__ idx2, uint64_t n_perm,
                                                  uint64_t perm_base, uint64_t seed, uint32_t job_id,
@@ -1669,7 +1673,8 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
       s2 = shuffle_make(seed, job_id, (uint32_t)pid, 2u, J.N2);
     }
     BestKey b =
-        optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS, NBL>(J, Lo, S, i1, i2, s1, s2, err, st, gpairs, &nt);
+        optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS, NBL, ZC ? 1 : 0>(J, Lo, S, i1, i2, s1, s2, err, st,
+                                                                           gpairs, &nt);
     /* a counter value claimed during this task (optimize_perm claims one
      * every time, after the build): published by its last barrier, and new,
      * so the loop always moves on */
