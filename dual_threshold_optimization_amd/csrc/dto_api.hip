@@ -273,11 +273,11 @@ struct StagingOrder {
   }
 };
 
-template <bool E, bool HL, bool PL, bool PA, bool RC, bool NBL, bool ZC>
-static hipError_t launch_opt_z(const dto_job *j, const LdsLayout &lo, uint32_t grid, hipStream_t st,
-                               const uint32_t *i1, const uint32_t *i2, uint64_t n_perm, uint64_t base, uint64_t seed,
-                               uint32_t job_id, int32_t permuted, dto_record *out) {
-  auto kern = k_optimize<kNT, E, HL, PL, PA, RC, NBL, ZC>;
+template <bool E, bool HL, bool PL, bool PA, bool RC, bool NBL = true>
+static hipError_t launch_opt(const dto_job *j, const LdsLayout &lo, uint32_t grid, hipStream_t st,
+                             const uint32_t *i1, const uint32_t *i2, uint64_t n_perm, uint64_t base, uint64_t seed,
+                             uint32_t job_id, int32_t permuted, dto_record *out) {
+  auto kern = k_optimize<kNT, E, HL, PL, PA, RC, NBL>;
   (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lo.total);
   /* a slot of the job's counter ring for this launch, zeroed on its stream */
   uint32_t *ctr = j->d_ctr + (const_cast<dto_job *>(j)->ctr_next.fetch_add(1u) % kCounterRing);
@@ -286,21 +286,6 @@ static hipError_t launch_opt_z(const dto_job *j, const LdsLayout &lo, uint32_t g
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kNT), lo.total, st, j->dev, lo, i1, i2, n_perm, base, seed, job_id,
                      permuted, out, j->d_err, j->ws, j->gnb, ctr);
   return hipGetLastError();
-}
-
-/* The screened kernels with the table in LDS and the row in registers leave
- * the k > K test out for jobs where no overlap can exceed K (DevJob.kle);
- * the other shapes keep it (one kernel fewer each). */
-template <bool E, bool HL, bool PL, bool PA, bool RC, bool NBL = true>
-static hipError_t launch_opt(const dto_job *j, const LdsLayout &lo, uint32_t grid, hipStream_t st,
-                             const uint32_t *i1, const uint32_t *i2, uint64_t n_perm, uint64_t base, uint64_t seed,
-                             uint32_t job_id, int32_t permuted, dto_record *out) {
-  if constexpr (!PA && HL && RC) {
-    if (j->dev.kle)
-      return launch_opt_z<E, HL, PL, PA, RC, NBL, false>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted,
-                                                          out);
-  }
-  return launch_opt_z<E, HL, PL, PA, RC, NBL, true>(j, lo, grid, st, i1, i2, n_perm, base, seed, job_id, permuted, out);
 }
 
 /* Lane segments fit in registers when a row is at most kSegWords words per lane. */

@@ -1637,9 +1637,9 @@ __device__ __forceinline__ void write_record(const DevJob &J, const BestKey &b, 
  * workgroup pair staging (N2 u16 each) when !PL.  Workgroups take
  * permutations from a device counter (zeroed before the launch), so that
  * permutations of uneven cost spread evenly; every workgroup exits once the
- * counter passes n_perm.  ZC false: the job cannot have cells with k > K
- * (DevJob.kle), and the screen leaves that test out. */
-template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS, bool NBL, bool ZC>
+ * counter passes n_perm.  The screen keeps its k > K test for every job:
+ * one code path in the sweeps costs less than leaving the test out. */
+template <int NT, bool EXPLICIT, bool HLDS, bool PLAIN, bool PL, bool REGCOLS, bool NBL>
 __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const uint32_t *__restrict__ idx1,
                                                  const uint32_t *__restrict__ idx2, uint64_t n_perm,
                                                  uint64_t perm_base, uint64_t seed, uint32_t job_id,
@@ -1673,8 +1673,7 @@ __global__ __launch_bounds__(NT) void k_optimize(DevJob J, LdsLayout Lo, const u
       s2 = shuffle_make(seed, job_id, (uint32_t)pid, 2u, J.N2);
     }
     BestKey b =
-        optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS, NBL, ZC ? 1 : 0>(J, Lo, S, i1, i2, s1, s2, err, st,
-                                                                           gpairs, &nt);
+        optimize_perm<NT, EXPLICIT, HLDS, PLAIN, PL, REGCOLS, NBL, 1>(J, Lo, S, i1, i2, s1, s2, err, st, gpairs, &nt);
     /* a counter value claimed during this task (optimize_perm claims one
      * every time, after the build): published by its last barrier, and new,
      * so the loop always moves on */
