@@ -361,6 +361,12 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
   for (uint32_t t = tid; t <= T1; t += NT) S.rowstart[t] = 0;
   for (uint32_t t = tid; t < W * T2wp; t += NT) S.cw[t] = 0;
   if (tid == 0) *S.ushare = fkey(__builtin_inff()); /* published after the build's barriers */
+  /* the tables this build reads, each held as its own scalar pair: read
+   * straight from the job they are one spilled tuple with the job's other
+   * pointers, reloaded whole (16 lane reads) at every use */
+  const uint16_t *bin1 = J.bin1, *bin2 = J.bin2;
+  const int32_t *join = J.join;
+  asm("" : "+s"(bin1), "+s"(bin2), "+s"(join));
   /* A: list1 feature m sits at the position k with idx1[k] == m (loads of
    * kU positions in flight per thread) */
   constexpr uint32_t kU = 16;
@@ -368,7 +374,7 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
     shuffle_visit<NT>(s1, S.redS, [&](const uint32_t *k, const uint32_t *m) {
       uint32_t bn[kShufU];
 #pragma unroll
-      for (uint32_t u = 0; u < kShufU; ++u) bn[u] = m[u] < J.N1 ? J.bin1[k[u]] : 0u;
+      for (uint32_t u = 0; u < kShufU; ++u) bn[u] = m[u] < J.N1 ? bin1[k[u]] : 0u;
 #pragma unroll
       for (uint32_t u = 0; u < kShufU; ++u)
         if (m[u] < J.N1) nb_st<NBL>(S, m[u], (uint16_t)bn[u]);
@@ -383,7 +389,7 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
         bn[u] = 0u;
         if (k < J.N1) {
           m[u] = i1 ? i1[k] : k;
-          bn[u] = J.bin1[k];
+          bn[u] = bin1[k];
         }
       }
 #pragma unroll
@@ -431,8 +437,8 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
 #pragma unroll
       for (uint32_t u = 0; u < kShufU; ++u) {
         const bool in = q[u] < J.N2;
-        b[u] = in ? J.bin2[e[u]] : kBinNone;
-        jm[u] = (in && !J.dup) ? J.join[q[u]] : -1;
+        b[u] = in ? bin2[e[u]] : kBinNone;
+        jm[u] = (in && !J.dup) ? join[q[u]] : -1;
       }
 #pragma unroll
       for (uint32_t u = 0; u < kShufU; ++u)
@@ -449,11 +455,11 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
         b[u] = kBinNone;
         if (e < J.N2) {
           q[u] = i2 ? i2[e] : e;
-          b[u] = J.bin2[e];
+          b[u] = bin2[e];
         }
       }
 #pragma unroll
-      for (uint32_t u = 0; u < kU; ++u) jm[u] = (q[u] < J.N2 && !J.dup) ? J.join[q[u]] : -1;
+      for (uint32_t u = 0; u < kU; ++u) jm[u] = (q[u] < J.N2 && !J.dup) ? join[q[u]] : -1;
 #pragma unroll
       for (uint32_t u = 0; u < kU; ++u) {
         const uint32_t e = e0 + u * NT;
@@ -488,7 +494,7 @@ __device__ __forceinline__ void build_rows(const DevJob &J, const LdsLayout &Lo,
           b[u] = (uint16_t)(pr & 0xFFFFu);
         } else {
           a[u] = PL ? lpairs[e] : vpairs[e];
-          b[u] = J.bin2[e];
+          b[u] = bin2[e];
         }
       }
     }
