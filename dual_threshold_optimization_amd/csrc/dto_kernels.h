@@ -1073,10 +1073,10 @@ __device__ __forceinline__ void segment_fast(const DevJob &J, const Lds &S, cons
 
 /* flush_first's tier-1 test, run on a segment's first-term survivors where
  * they sit in registers: a segment with more survivors than the first-term
- * list holds is tested here rather than through passes of flush and rescan.
- * The cells are row i's (constants K, and x as the screen computed it, which
- * is the value flush_first rebuilds); survivors flagged in surv, column of
- * cell c is j0 + c.  Tier-1 survivors go to the tier-2 list. */
+ * list holds is tested here instead of through the list.  The cells are row
+ * i's (constants K, and x as the screen computed it, which is the value
+ * flush_first rebuilds); survivors flagged in surv, column of cell c is
+ * j0 + c.  Tier-1 survivors go to the tier-2 list. */
 template <int NT>
 __device__ __forceinline__ void tier1_segment(const DevJob &J, const Lds &S, uint32_t surv, const uint32_t *kk,
                                               const uint32_t *nn4, const float *xx, uint32_t K, uint32_t i,
