@@ -974,6 +974,33 @@ struct ColRegs {
   uint32_t wa[kSegWords]; /* LDS byte addresses of the lane's count words */
 };
 
+/* Lower tail.  The first term is a useless lower bound of p for a cell below
+ * its mean, where p is close to 1 but P(X = k) can be as small as the upper
+ * tail's terms.  Cantelli's inequality, P(X <= mu - lam) <= var / (var +
+ * lam^2) for lam > 0, gives P(X >= k) >= 1/2 whenever lam = mu - k + 1 >=
+ * max(sigma, 1).  With sigma' >= max(sigma_ij, 1) over a lane's cells of a
+ * row and D4 >= 4 N (sigma' - 1), the test 4 k N + D4 <= 4 K n implies it
+ * (k <= mu + 1 - sigma').  Valid while 4 N^2 + D4 < 2^32 (N <= kLiftMaxN);
+ * larger jobs get no lift (c = -inf). */
+constexpr uint32_t kLiftMaxN = 32000;
+constexpr float kLogHalfBelow = -0.69315f; /* below log(1/2) */
+
+/* D4 for row constant K: var_ij = K (N - K) / N^2 * n (N - n) / (N - 1) and
+ * Vl bounds the last factor over the lane's columns. */
+__device__ __forceinline__ uint32_t lift_offset(uint32_t K, uint32_t N, float Nf, float invN2, float Vl) {
+  const float s = (float)K * (float)(N - K) * invN2;
+  const float sg = fmaxf(__builtin_sqrtf(s * Vl) * 1.001f + 1e-3f, 1.f);
+  return (uint32_t)ceilf(fminf((sg - 1.f) * Nf + 1.f, 1e9f)) << 2;
+}
+
+/* A cell's screen value raised to log(1/2) (c) when the test above holds.
+ * p == 0 cells (k > K >= mu) and p == 1 cells (x ~ +3e38) are unchanged. */
+__device__ __forceinline__ float lift_lower(float x, uint32_t k, uint32_t n4, uint32_t K, uint32_t N4, uint32_t D4,
+                                            float c) {
+  const bool below = __umul24(k, N4) + D4 <= __umul24(n4, K);
+  return fmaxf(x, below ? c : -__builtin_inff());
+}
+
 /* LDS byte address of a pointer into the dynamic region. */
 __device__ __forceinline__ uint32_t lds_addr(const void *p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
@@ -1269,6 +1296,11 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   /* this lane's column constants, kept in registers across rows; columns
    * past T2 repeat the last column's n (their k equals its k) and get no cell */
   ColRegs cr;
+  /* the lower-tail lift (REGCOLS): N n (N - n) / (N - 1) bound over the lane's columns */
+  const float liftNf = (float)J.N, liftInvN2 = 1.f / (liftNf * liftNf);
+  const uint32_t liftN4 = J.N << 2;
+  const float liftc = J.N <= kLiftMaxN ? kLogHalfBelow : -__builtin_inff();
+  float liftV = 0.f;
   if (REGCOLS) {
 #pragma unroll
     for (uint32_t c = 0; c < kSegCells; ++c) {
@@ -1277,7 +1309,10 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       const uint32_t jc = valid ? j : T2 - 1u;
       cr.n4[c] = S.coln[jc] << 2;
       cr.b[c] = valid ? S.colb[jc] : kNoCell;
+      const float nf = (float)S.coln[jc];
+      liftV = fmaxf(liftV, nf * (liftNf - nf));
     }
+    liftV = J.N > 1u ? liftV / (liftNf - 1.f) : 0.f;
 #pragma unroll
     for (uint32_t u = 0; u < kSegWords; ++u) {
       const uint32_t last = Lo.T2w_pad - 1u;
@@ -1337,6 +1372,13 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
         segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
       else
         segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
+      if (REGCOLS) {
+        /* row minima and the lane's best cell from the lifted values (a
+         * lifted best cell only loosens its own upper bound: x rises) */
+        const uint32_t D4 = lift_offset(K, J.N, liftNf, liftInvN2, liftV);
+#pragma unroll
+        for (uint32_t c = 0; c < kSegCells; ++c) xx[c] = lift_lower(xx[c], kk[c], nn4[c], K, liftN4, D4, liftc);
+      }
       if (t0 == 0u && inext < T1) {
         /* the next update's pairs: rows [i + 1, inext + 1) */
         pe0 = pe1;
@@ -1494,8 +1536,16 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
          * is below -708 (possibly subnormal or zero sums) always pass */
         const float thr = fmaxf(Uw + margin, -708.f + margin);
         uint32_t surv = 0;
+        if (REGCOLS) {
+          /* lifted lower-tail cells do not pass; xx keeps the first-term values */
+          const uint32_t D4 = lift_offset(K, J.N, liftNf, liftInvN2, liftV);
 #pragma unroll
-        for (uint32_t c = 0; c < kSegCells; ++c) surv |= xx[c] <= thr ? 1u << c : 0u;
+          for (uint32_t c = 0; c < kSegCells; ++c)
+            surv |= lift_lower(xx[c], kk[c], nn4[c], K, liftN4, D4, liftc) <= thr ? 1u << c : 0u;
+        } else {
+#pragma unroll
+          for (uint32_t c = 0; c < kSegCells; ++c) surv |= xx[c] <= thr ? 1u << c : 0u;
+        }
         if (!__ballot(surv != 0u)) continue;
         st.cheap += (uint32_t)__popc(surv);
         uint32_t stot = 0;
