@@ -979,25 +979,30 @@ struct ColRegs {
  * tail's terms.  Cantelli's inequality, P(X <= mu - lam) <= var / (var +
  * lam^2) for lam > 0, gives P(X >= k) >= 1/2 whenever lam = mu - k + 1 >=
  * max(sigma, 1).  With sigma' >= max(sigma_ij, 1) over a lane's cells of a
- * row and D4 >= 4 N (sigma' - 1), the test 4 k N + D4 <= 4 K n implies it
- * (k <= mu + 1 - sigma').  Valid while 4 N^2 + D4 < 2^32 (N <= kLiftMaxN);
- * larger jobs get no lift (c = -inf). */
+ * row and D >= N (sigma' - 1), the test k N + D <= K n implies it (k <= mu +
+ * 1 - sigma').  Rows held in registers test 4 k N + 4 D <= K (4 n), valid
+ * while 4 N^2 + 4 D < 2^32 (N <= kLiftMaxN); wider rows test k N + D <= K n,
+ * valid while N^2 + D < 2^32 (N <= kLiftMaxN1).  Larger jobs get no lift
+ * (c = -inf). */
 constexpr uint32_t kLiftMaxN = 32000;
+constexpr uint32_t kLiftMaxN1 = 65000;
 constexpr float kLogHalfBelow = -0.69315f; /* below log(1/2) */
 
-/* D4 for row constant K: var_ij = K (N - K) / N^2 * n (N - n) / (N - 1) and
- * Vl bounds the last factor over the lane's columns. */
-__device__ __forceinline__ uint32_t lift_offset(uint32_t K, uint32_t N, float Nf, float invN2, float Vl) {
+/* D for row constant K: var_ij = K (N - K) / N^2 * n (N - n) / (N - 1) and
+ * V bounds the last factor over the lane's columns. */
+__device__ __forceinline__ uint32_t lift_offset(uint32_t K, uint32_t N, float Nf, float invN2, float V) {
   const float s = (float)K * (float)(N - K) * invN2;
-  const float sg = fmaxf(__builtin_sqrtf(s * Vl) * 1.001f + 1e-3f, 1.f);
-  return (uint32_t)ceilf(fminf((sg - 1.f) * Nf + 1.f, 1e9f)) << 2;
+  const float sg = fmaxf(__builtin_sqrtf(s * V) * 1.001f + 1e-3f, 1.f);
+  return (uint32_t)ceilf(fminf((sg - 1.f) * Nf + 1.f, 1e9f));
 }
 
 /* A cell's screen value raised to log(1/2) (c) when the test above holds.
- * p == 0 cells (k > K >= mu) and p == 1 cells (x ~ +3e38) are unchanged. */
+ * p == 0 cells (k > K >= mu) and p == 1 cells (x ~ +3e38) are unchanged.
+ * X4: the scaled test (N4 = 4 N, D4 = 4 D), else the plain one (N4 = N, D4 = D). */
+template <bool X4>
 __device__ __forceinline__ float lift_lower(float x, uint32_t k, uint32_t n4, uint32_t K, uint32_t N4, uint32_t D4,
                                             float c) {
-  const bool below = __umul24(k, N4) + D4 <= __umul24(n4, K);
+  const bool below = __umul24(k, N4) + D4 <= __umul24(X4 ? n4 : n4 >> 2, K);
   return fmaxf(x, below ? c : -__builtin_inff());
 }
 
@@ -1296,11 +1301,12 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
   /* this lane's column constants, kept in registers across rows; columns
    * past T2 repeat the last column's n (their k equals its k) and get no cell */
   ColRegs cr;
-  /* the lower-tail lift (REGCOLS): N n (N - n) / (N - 1) bound over the lane's columns */
+  /* the lower-tail lift: the bound of n (N - n) / (N - 1) over the lane's
+   * columns (REGCOLS), or over all n (N^2 / 4 / (N - 1), wider rows) */
   const float liftNf = (float)J.N, liftInvN2 = 1.f / (liftNf * liftNf);
-  const uint32_t liftN4 = J.N << 2;
-  const float liftc = J.N <= kLiftMaxN ? kLogHalfBelow : -__builtin_inff();
-  float liftV = 0.f;
+  const uint32_t liftN4 = REGCOLS ? J.N << 2 : J.N;
+  const float liftc = J.N <= (REGCOLS ? kLiftMaxN : kLiftMaxN1) ? kLogHalfBelow : -__builtin_inff();
+  float liftV = !REGCOLS && J.N > 1u ? liftNf * liftNf / (4.f * (liftNf - 1.f)) * 1.001f : 0.f;
   if (REGCOLS) {
 #pragma unroll
     for (uint32_t c = 0; c < kSegCells; ++c) {
@@ -1358,6 +1364,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
     const uint32_t NK4 = (uint32_t)__builtin_amdgcn_readlane((int)NKl, (int)(q & 63u));
     const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Al), (int)(q & 63u)));
     float lmin = __builtin_inff(); /* packed (value, cell) */
+    const uint32_t liftD = lift_offset(K, J.N, liftNf, liftInvN2, liftV) << (REGCOLS ? 2 : 0);
     uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
     /* REGCOLS: the row is one segment */
 #pragma unroll 1
@@ -1372,12 +1379,12 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
         segment_fast<HLDS, REGCOLS, false>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
       else
         segment_fast<HLDS, REGCOLS, true>(J, S, Hb, wv, nseg, wbeg + t0, run, K, NK4, A, cr, kk, nn4, xx);
-      if (REGCOLS) {
+      {
         /* row minima and the lane's best cell from the lifted values (a
          * lifted best cell only loosens its own upper bound: x rises) */
-        const uint32_t D4 = lift_offset(K, J.N, liftNf, liftInvN2, liftV);
 #pragma unroll
-        for (uint32_t c = 0; c < kSegCells; ++c) xx[c] = lift_lower(xx[c], kk[c], nn4[c], K, liftN4, D4, liftc);
+        for (uint32_t c = 0; c < kSegCells; ++c)
+          xx[c] = lift_lower<REGCOLS>(xx[c], kk[c], nn4[c], K, liftN4, liftD, liftc);
       }
       if (t0 == 0u && inext < T1) {
         /* the next update's pairs: rows [i + 1, inext + 1) */
@@ -1517,6 +1524,7 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
       const uint32_t K = (uint32_t)__builtin_amdgcn_readlane((int)Kl, (int)l);
       const uint32_t NK4 = (uint32_t)__builtin_amdgcn_readlane((int)NKl, (int)l);
       const float A = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Al), (int)l));
+      const uint32_t liftD = lift_offset(K, J.N, liftNf, liftInvN2, liftV) << (REGCOLS ? 2 : 0);
       uint32_t run = REGCOLS ? 0u : row_lane_offset(C, wbeg, c2, T2w);
 #pragma unroll 1
       for (uint32_t t0 = 0; REGCOLS ? t0 == 0u : t0 < c2; t0 += kSegWords) {
@@ -1536,16 +1544,10 @@ __device__ __forceinline__ BestKey optimize_perm(const DevJob &J, const LdsLayou
          * is below -708 (possibly subnormal or zero sums) always pass */
         const float thr = fmaxf(Uw + margin, -708.f + margin);
         uint32_t surv = 0;
-        if (REGCOLS) {
-          /* lifted lower-tail cells do not pass; xx keeps the first-term values */
-          const uint32_t D4 = lift_offset(K, J.N, liftNf, liftInvN2, liftV);
+        /* lifted lower-tail cells do not pass; xx keeps the first-term values */
 #pragma unroll
-          for (uint32_t c = 0; c < kSegCells; ++c)
-            surv |= lift_lower(xx[c], kk[c], nn4[c], K, liftN4, D4, liftc) <= thr ? 1u << c : 0u;
-        } else {
-#pragma unroll
-          for (uint32_t c = 0; c < kSegCells; ++c) surv |= xx[c] <= thr ? 1u << c : 0u;
-        }
+        for (uint32_t c = 0; c < kSegCells; ++c)
+          surv |= lift_lower<REGCOLS>(xx[c], kk[c], nn4[c], K, liftN4, liftD, liftc) <= thr ? 1u << c : 0u;
         if (!__ballot(surv != 0u)) continue;
         st.cheap += (uint32_t)__popc(surv);
         uint32_t stot = 0;

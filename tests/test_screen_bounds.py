@@ -3,7 +3,9 @@
 The screened kernels raise a cell's lower bound of log p to log(1/2) when an
 integer test says the cell lies at least max(sigma, 1) - 1 below its mean
 (lift_offset / lift_lower in csrc/dto_kernels.h); Cantelli's inequality then
-gives P(X >= k) >= 1/2.  Here the same test, in the same float32 and u32
+gives P(X >= k) >= 1/2.  Rows held in registers use a scaled test (4 k N +
+4 D <= K 4n) and a per-lane variance bound; wider rows the plain test (k N +
+D <= K n) and the bound over all n.  Here the same test, in the same float32 and u32
 arithmetic, is run over whole (N, K, n) grids and every cell it lifts is
 checked against scipy's hypergeometric tail.  The check fails a test that
 lifts cells at or above the median, e.g. one whose offset D4 moves the bound
@@ -13,7 +15,8 @@ import numpy as np
 import pytest
 from scipy.stats import hypergeom
 
-LIFT_MAX_N = 32000  # kLiftMaxN
+LIFT_MAX_N = 32000  # kLiftMaxN (scaled test)
+LIFT_MAX_N1 = 65000  # kLiftMaxN1 (plain test)
 
 
 def lane_var_bound(N, ns):
@@ -24,29 +27,36 @@ def lane_var_bound(N, ns):
     return np.float32(v / (Nf - np.float32(1))) if N > 1 else np.float32(0)
 
 
+def row_var_bound(N):
+    """The wide rows' bound of n (N - n) / (N - 1) over all n (float32)."""
+    Nf = np.float32(N)
+    return np.float32(np.float32(Nf * Nf / (np.float32(4) * (Nf - np.float32(1)))) * np.float32(1.001))
+
+
 def lift_offset(K, N, V):
-    """D4 of the kernel: 4 ceil(N (sigma' - 1) + 1), sigma' >= max(sigma, 1)."""
+    """D of the kernel: ceil(N (sigma' - 1) + 1), sigma' >= max(sigma, 1)."""
     Nf = np.float32(N)
     invN2 = np.float32(1) / (Nf * Nf)
     s = np.float32(K) * np.float32(N - K) * invN2
     sg = np.float32(np.sqrt(np.float32(s * V)))
     sg = max(np.float32(sg * np.float32(1.001) + np.float32(1e-3)), np.float32(1))
     d = np.float32(min(np.float32((sg - np.float32(1)) * Nf + np.float32(1)), np.float32(1e9)))
-    return (int(np.ceil(d)) << 2) & 0xFFFFFFFF
+    return int(np.ceil(d))
 
 
-def lifted_cells(N, K, n, V, offset_side=1):
-    """Overlaps k in [0, min(K, n)] that the kernel's test lifts
-    (offset_side -1: the planted mistake, D4 on the other side)."""
-    D4 = lift_offset(K, N, V)
+def lifted_cells(N, K, n, V, offset_side=1, scale=4):
+    """Overlaps k in [0, min(K, n)] that the kernel's test lifts (scale 4:
+    rows in registers, 1: wider rows; offset_side -1: the planted mistake,
+    D on the other side)."""
+    D = (lift_offset(K, N, V) * scale) & 0xFFFFFFFF
     k = np.arange(0, min(K, n) + 1, dtype=np.uint64)
-    lhs = (k * np.uint64(4 * N) + np.uint64(D4 if offset_side > 0 else 0)) & np.uint64(0xFFFFFFFF)
-    rhs = np.uint64(4 * n * K + (D4 if offset_side < 0 else 0)) & np.uint64(0xFFFFFFFF)
+    lhs = (k * np.uint64(scale * N) + np.uint64(D if offset_side > 0 else 0)) & np.uint64(0xFFFFFFFF)
+    rhs = np.uint64(scale * n * K + (D if offset_side < 0 else 0)) & np.uint64(0xFFFFFFFF)
     return k[lhs <= rhs].astype(np.int64)
 
 
-def violations(N, K, n, V, offset_side=1):
-    k = lifted_cells(N, K, n, V, offset_side)
+def violations(N, K, n, V, offset_side=1, scale=4):
+    k = lifted_cells(N, K, n, V, offset_side, scale)
     if k.size == 0:
         return 0, 0
     p = hypergeom.sf(k - 1, N, K, n)  # P(X >= k)
@@ -98,9 +108,20 @@ def test_lower_tail_lift_check_catches_a_wrong_side_offset():
     assert bad > 0
 
 
-@pytest.mark.parametrize("N", [LIFT_MAX_N])
-def test_lower_tail_lift_integer_range(N):
-    # 4 k N + D4 stays below 2^32 for every k <= N at the largest lifted N
-    V = lane_var_bound(N, [N // 2])
-    D4 = max(lift_offset(K, N, V) for K in range(1, N + 1, 97))
-    assert 4 * N * N + D4 < 2 ** 32
+def test_lower_tail_lift_wide_rows_is_a_valid_bound():
+    lifted = 0
+    cases = grid_cases(11, 6) + [(N, K, n) for N in (40000, LIFT_MAX_N1)
+                                 for K, n in ((N // 2, N // 2), (N // 3, N // 7), (N - 5, 17), (N // 50, N // 2))]
+    for N, K, n in cases:
+        bad, cnt = violations(N, K, n, row_var_bound(N), scale=1)
+        assert bad == 0, (N, K, n)
+        lifted += cnt
+    assert lifted > 10000
+
+
+@pytest.mark.parametrize("N,scale", [(LIFT_MAX_N, 4), (LIFT_MAX_N1, 1)])
+def test_lower_tail_lift_integer_range(N, scale):
+    # scale (k N + D) stays below 2^32 for every k <= N at the largest lifted N
+    V = max(lane_var_bound(N, [N // 2]), row_var_bound(N))
+    D = max(lift_offset(K, N, V) for K in range(1, N + 1, 97))
+    assert scale * (N * N + D) < 2 ** 32
