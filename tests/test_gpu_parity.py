@@ -260,6 +260,27 @@ def test_screened_equals_plain_many_permutations(dto, gpu):
     assert [rec_tuple(x) for x in es] == [rec_tuple(x) for x in ep]
 
 
+def test_screened_equals_plain_weak_minima(dto, gpu):
+    """Uncorrelated lists have weak minima, where cells below their means
+    crowd the screen and the lower-tail bound decides most rescans.  Screened
+    records must equal the plain FP64 ones, both for rows held in registers
+    (reference thresholds) and for wide rows (over 640 custom thresholds)."""
+    rng = np.random.default_rng(41)
+    for case, n in enumerate((300, 900, 2500, 6000)):
+        ids1, r1, ids2, r2 = random_lists(rng, n, tie_frac=0.1 * (case % 2), extra=n // 10)
+        N = len(set(ids1) | set(ids2))
+        job = dto.Job.with_population(ids1, r1, ids2, r2, N)
+        for wide in (False, True):
+            if wide:
+                L = len(ids1)
+                t = np.arange(1, L + 1, max(1, L // 1200), dtype=np.uint32)
+                job.set_thresholds(t, t)
+            P = 384
+            _, ps = job.run(P, seed=100 + case, mode="screened")
+            _, pp = job.run(P, seed=100 + case, mode="plain")
+            assert [rec_tuple(x) for x in ps] == [rec_tuple(x) for x in pp], (n, wide)
+
+
 def test_dense_thresholds_vs_oracle(dto, O, gpu):
     """Custom threshold grids wider than one register segment per lane
     (column constants read from LDS), against the oracle with the same
